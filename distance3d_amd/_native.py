@@ -1,0 +1,148 @@
+"""ctypes binding of libd3d_hip.so (the C ABI declared in include/d3d.h).
+
+The library is the only compute path of this package: every public function
+that computes a support point, an AABB, a GJK/EPA result or a broad-phase pair
+set calls into it.  There is no CPU fallback; when the library or a GPU is
+missing, :func:`lib` raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libd3d_hip.so")
+
+# collider types (include/d3d.h)
+HULL, SPHERE, CAPSULE, ELLIPSOID, CYLINDER, BOX, CONE, DISK, ELLIPSE = range(9)
+
+# GJK status codes
+SEPARATED = 0
+INTERSECT = 1
+CLIPPED = 3
+ITER_CAP = 4
+SANITY_FAILED = 0x100
+BAD_INPUT = 0x200
+
+
+class D3DError(RuntimeError):
+    """A libd3d_hip entry point returned an error code."""
+
+
+class d3d_scene(ctypes.Structure):
+    _fields_ = [
+        ("shape_f64", ctypes.c_void_p),
+        ("shape_i32", ctypes.c_void_p),
+        ("vx", ctypes.c_void_p),
+        ("vy", ctypes.c_void_p),
+        ("vz", ctypes.c_void_p),
+        ("v64", ctypes.c_void_p),
+        ("n_shapes", ctypes.c_int64),
+        ("n_vertices", ctypes.c_int64),
+        ("n_vertices64", ctypes.c_int64),
+    ]
+
+
+class d3d_gjk_params(ctypes.Structure):
+    _fields_ = [
+        ("tolerance", ctypes.c_double),
+        ("max_distance_squared", ctypes.c_double),
+        ("sanity_check", ctypes.c_double),
+        ("max_iterations", ctypes.c_int32),
+        ("hull_serial_max", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_D = ctypes.c_double
+_SCENE = ctypes.POINTER(d3d_scene)
+
+# name -> argtypes; every function returns int
+_SIGNATURES = {
+    "d3d_support_batch": [_SCENE, _P, _P, _I64, _P, _P],
+    "d3d_aabb_batch": [_SCENE, _P, _I64, _P, _P],
+    "d3d_gjk_distance_batch": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params),
+                               _P, _P, _P, _P, _P, _P, _P, _P],
+    "d3d_gjk_intersection_batch": [_SCENE, _P, _P, _I64, _D, _I32, _P, _P, _P],
+}
+
+_lock = threading.Lock()
+_lib = None
+_initialised = False
+
+
+def exported_symbols():
+    """Names of the C-ABI functions this binding uses."""
+    return ["d3d_abi_version", "d3d_last_error", "d3d_init"] + list(_SIGNATURES)
+
+
+def load_library(path=LIB_PATH):
+    """Load libd3d_hip.so without touching the GPU (no HIP call is made)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise D3DError(
+                    "%s is missing: build it with __graft_entry__.build() or "
+                    "`make -C distance3d_amd/csrc`" % path)
+            lib = ctypes.CDLL(path)
+            lib.d3d_abi_version.restype = ctypes.c_int
+            lib.d3d_last_error.restype = ctypes.c_char_p
+            lib.d3d_init.argtypes = [ctypes.c_int]
+            lib.d3d_init.restype = ctypes.c_int
+            for name, argtypes in _SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = argtypes
+                fn.restype = ctypes.c_int
+            _lib = lib
+        return _lib
+
+
+def lib():
+    """The loaded library, initialised on the current torch CUDA device."""
+    global _initialised
+    import torch
+    l = load_library()
+    if not _initialised:
+        if not torch.cuda.is_available():
+            raise D3DError("distance3d_amd needs a ROCm GPU (gfx950); none is visible")
+        check(l.d3d_init(torch.cuda.current_device()), "d3d_init")
+        _initialised = True
+    return l
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = _lib.d3d_last_error().decode() if _lib is not None else ""
+        raise D3DError("%s failed (code %d): %s" % (what, rc, msg))
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def gjk_params(tolerance=1e-10, max_distance_squared=100000.0, sanity_check=1e-8,
+               max_iterations=128, hull_serial_max=16):
+    p = d3d_gjk_params()
+    p.tolerance = float(tolerance)
+    p.max_distance_squared = float(max_distance_squared)
+    p.sanity_check = float(sanity_check)
+    p.max_iterations = int(max_iterations)
+    p.hull_serial_max = int(hull_serial_max)
+    return p
+
+
+EPSILON = float(np.finfo(float).eps)
+MAX_FLOAT = float(np.finfo(float).max)

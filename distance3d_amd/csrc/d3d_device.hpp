@@ -1,0 +1,661 @@
+// d3d_device.hpp -- device-side geometry for the gfx950 collision kernels:
+// FP64 vector math, collider records, support functions (analytic in FP64,
+// convex-hull argmax as an FP32 wavefront sweep with an exactness
+// certificate), AABBs, and the Jolt/van den Bergen simplex sub-routines.
+//
+// Precision contract: every quantity that reaches the simplex solve, the
+// termination tests or the closest-point reconstruction is FP64, evaluated
+// left to right with one rounding per operation (the library is compiled with
+// -ffp-contract=off).  Only the hull argmax scan runs in FP32, and its winner is
+// certified against the FP64 argmax (see coop_hull_argmax).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "d3d.h"
+
+namespace d3d {
+
+constexpr double kEps = 2.220446049250313e-16;  // utils.py:8 (np.finfo(float).eps)
+constexpr double kEpsSqr = kEps * kEps;         // gjk/_gjk_jolt.py:18
+constexpr double kMaxFloat = 1.7976931348623157e308;  // utils.py:7
+constexpr int kWave = 64;
+
+struct v3 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ v3 mk(double x, double y, double z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 operator-(v3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ v3 operator*(v3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// scalar_triple_product (utils.py:33-73)
+__device__ __forceinline__ double stp(v3 a, v3 b, v3 c) { return dot(a, cross(b, c)); }
+
+// norm_vector (utils.py:12-30)
+__device__ __forceinline__ v3 norm_vector(v3 v) {
+    double n = sqrt(dot(v, v));
+    if (n == 0.0) return v;
+    return mk(v.x / n, v.y / n, v.z / n);
+}
+
+// ------------------------------------------------------------ wave helpers
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ int readlane_i32(int v, int lane) {
+    return __builtin_amdgcn_readlane(v, lane);
+}
+
+// ------------------------------------------------------------- colliders
+
+// Collider record as laid out in the scene (see d3d.h).
+struct Shape {
+    double R[9];
+    double t[3];
+    double p[3];
+    double margin;
+    int type, nv, voff, voff64;
+};
+
+__device__ __forceinline__ Shape load_shape(const d3d_scene &s, int k) {
+    Shape h;
+    const double *f = s.shape_f64 + 16 * (int64_t)k;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) h.R[j] = f[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        h.t[j] = f[9 + j];
+        h.p[j] = f[12 + j];
+    }
+    h.margin = f[15];
+    const int4 I = *reinterpret_cast<const int4 *>(s.shape_i32 + 4 * (int64_t)k);
+    h.type = I.x;
+    h.nv = I.y;
+    h.voff = I.z;
+    h.voff64 = I.w;
+    return h;
+}
+
+// R^T d, i.e. np.dot(A2B[:3, :3].T, d) (geometry.py:202)
+__device__ __forceinline__ v3 to_local(const Shape &h, v3 d) {
+    return mk(h.R[0] * d.x + h.R[3] * d.y + h.R[6] * d.z,
+              h.R[1] * d.x + h.R[4] * d.y + h.R[7] * d.z,
+              h.R[2] * d.x + h.R[5] * d.y + h.R[8] * d.z);
+}
+
+// transform_point: t + R p (utils.py:125-143)
+__device__ __forceinline__ v3 to_world(const Shape &h, v3 p) {
+    return mk(h.t[0] + (h.R[0] * p.x + h.R[1] * p.y + h.R[2] * p.z),
+              h.t[1] + (h.R[3] * p.x + h.R[4] * p.y + h.R[5] * p.z),
+              h.t[2] + (h.R[6] * p.x + h.R[7] * p.y + h.R[8] * p.z));
+}
+
+__device__ __forceinline__ v3 hull_vertex(const d3d_scene &s, int voff, int voff64, int idx) {
+    if (voff64 >= 0) {
+        const double *q = s.v64 + 3 * ((int64_t)voff64 + idx);
+        return mk(q[0], q[1], q[2]);
+    }
+    const int64_t j = (int64_t)voff + idx;
+    return mk((double)s.vx[j], (double)s.vy[j], (double)s.vz[j]);
+}
+
+// Lane-serial FP64 argmax over one hull: vertices[argmax(vertices . l)] with
+// the first index on ties (colliders.py:131-132, np.argmax).
+__device__ __forceinline__ int serial_hull_argmax(const d3d_scene &s, int voff, int voff64, int nv,
+                                                  v3 l) {
+    double best = -INFINITY;
+    int bi = 0;
+    for (int k = 0; k < nv; ++k) {
+        v3 v = hull_vertex(s, voff, voff64, k);
+        double val = v.x * l.x + v.y * l.y + v.z * l.z;
+        if (val > best) {
+            best = val;
+            bi = k;
+        }
+    }
+    return bi;
+}
+
+// Wavefront-cooperative argmax over one hull.  All 64 lanes call this with
+// the same (wave-uniform) arguments.  Each lane scans vertices lane, lane+64,
+// ... of the FP32 SoA pool (coalesced 256-byte rows), tracking the largest
+// score m1 (first index i1) and the largest score m2 among all other
+// vertices; a butterfly reduction merges the lanes.  The FP32 score of vertex
+// v differs from the exact v . l by at most
+//     E = 8 * 2^-24 * L * |l|_inf,   L = max_i (|x_i| + |y_i| + |z_i|)
+// (rounding of l to FP32, of the vertex when an FP64 pool backs it, and of
+// the three FMA steps, with a factor-of-two margin).  If m1 - m2 > 2E, vertex
+// i1 is the unique exact argmax, hence also the FP64 argmax.  Otherwise the
+// hull is rescanned in FP64 with the serial rule (first index on ties).
+__device__ __forceinline__ int coop_hull_argmax(const d3d_scene &s, int voff, int voff64, int nv,
+                                                double L, v3 l) {
+    const int lane = lane_id();
+    const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
+    const float *__restrict__ px = s.vx + voff;
+    const float *__restrict__ py = s.vy + voff;
+    const float *__restrict__ pz = s.vz + voff;
+    float m1 = -INFINITY, m2 = -INFINITY;
+    int i1 = 0x7fffffff;
+    int k = lane;
+    for (; k + kWave < nv; k += 2 * kWave) {
+        const float x0 = px[k], y0 = py[k], z0 = pz[k];
+        const float x1 = px[k + kWave], y1 = py[k + kWave], z1 = pz[k + kWave];
+        const float s0 = __builtin_fmaf(z0, dz, __builtin_fmaf(y0, dy, x0 * dx));
+        const float s1 = __builtin_fmaf(z1, dz, __builtin_fmaf(y1, dy, x1 * dx));
+        m2 = fmaxf(m2, fminf(s0, m1));
+        if (s0 > m1) { m1 = s0; i1 = k; }
+        m2 = fmaxf(m2, fminf(s1, m1));
+        if (s1 > m1) { m1 = s1; i1 = k + kWave; }
+    }
+    if (k < nv) {
+        const float s0 = __builtin_fmaf(pz[k], dz, __builtin_fmaf(py[k], dy, px[k] * dx));
+        m2 = fmaxf(m2, fminf(s0, m1));
+        if (s0 > m1) { m1 = s0; i1 = k; }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const float om1 = __shfl_xor(m1, off);
+        const float om2 = __shfl_xor(m2, off);
+        const int oi = __shfl_xor(i1, off);
+        m2 = fmaxf(fmaxf(m2, om2), fminf(m1, om1));
+        const bool take = (om1 > m1) || (om1 == m1 && oi < i1);
+        m1 = take ? om1 : m1;
+        i1 = take ? oi : i1;
+    }
+    const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
+    const double E = 8.0 * 5.9604644775390625e-08 * L * linf;
+    if ((double)m1 - (double)m2 > 2.0 * E) return i1;
+
+    double best = -INFINITY;
+    int bi = 0x7fffffff;
+    for (int j = lane; j < nv; j += kWave) {
+        v3 v = hull_vertex(s, voff, voff64, j);
+        double val = v.x * l.x + v.y * l.y + v.z * l.z;
+        if (val > best) {
+            best = val;
+            bi = j;
+        }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        const bool take = (ob > best) || (ob == best && oi < bi);
+        best = take ? ob : best;
+        bi = take ? oi : bi;
+    }
+    return bi == 0x7fffffff ? 0 : bi;
+}
+
+// Closed-form supports, FP64, in the same operation order as the reference.
+__device__ __forceinline__ v3 support_analytic(const Shape &h, v3 d) {
+    switch (h.type) {
+    case D3D_SPHERE: {  // geometry.py:314-346
+        const double r = h.p[0];
+        const double sn = sqrt(dot(d, d));
+        if (sn == 0.0) return mk(h.t[0] + 0.0, h.t[1] + 0.0, h.t[2] + r);
+        return mk(h.t[0] + d.x / sn * r, h.t[1] + d.y / sn * r, h.t[2] + d.z / sn * r);
+    }
+    case D3D_CAPSULE: {  // geometry.py:209-256
+        const v3 l = to_local(h, d);
+        const double r = h.p[0], ht = h.p[1];
+        const double s = sqrt(l.x * l.x + l.y * l.y + l.z * l.z);
+        v3 p;
+        if (s == 0.0) {
+            p = mk(r, 0.0, 0.0);
+        } else {
+            const double k = r / s;
+            p = mk(l.x * k, l.y * k, l.z * k);
+        }
+        if (l.z > 0.0) p.z += 0.5 * ht;
+        else p.z -= 0.5 * ht;
+        return to_world(h, p);
+    }
+    case D3D_ELLIPSOID: {  // geometry.py:259-284
+        const v3 l = to_local(h, d);
+        const v3 q = norm_vector(mk(l.x * h.p[0], l.y * h.p[1], l.z * h.p[2]));
+        return to_world(h, mk(q.x * h.p[0], q.y * h.p[1], q.z * h.p[2]));
+    }
+    case D3D_CYLINDER: {  // geometry.py:160-206
+        const v3 l = to_local(h, d);
+        const double r = h.p[0], L = h.p[1];
+        const double s = sqrt(l.x * l.x + l.y * l.y);
+        const double z = (l.z < 0.0) ? -0.5 * L : 0.5 * L;
+        v3 p;
+        if (s == 0.0) {
+            p = mk(r, 0.0, z);
+        } else {
+            const double k = r / s;
+            p = mk(l.x * k, l.y * k, z);
+        }
+        return to_world(h, p);
+    }
+    case D3D_BOX: {  // convert_box_to_vertices + argmax, geometry.py:135-157
+        const v3 l = to_local(h, d);
+        return to_world(h, mk((l.x > 0.0 ? 0.5 : -0.5) * h.p[0], (l.y > 0.0 ? 0.5 : -0.5) * h.p[1],
+                              (l.z > 0.0 ? 0.5 : -0.5) * h.p[2]));
+    }
+    case D3D_CONE: {  // geometry.py:417-454
+        const v3 l = to_local(h, d);
+        const double r = h.p[0], ht = h.p[1];
+        v3 dp = mk(l.x, l.y, 0.0);
+        const double n = sqrt(dot(dp, dp));
+        if (n == 0.0) {
+            dp = mk(0.0, 0.0, 0.0);
+        } else {
+            const double k = r / n;
+            dp = mk(dp.x * k, dp.y * k, dp.z * k);
+        }
+        const v3 p = (dot(l, dp) >= l.z * ht) ? dp : mk(0.0, 0.0, ht);
+        return to_world(h, p);
+    }
+    case D3D_DISK: {  // geometry.py:349-383
+        v3 l = to_local(h, d);
+        l.z = 0.0;
+        const double n = sqrt(dot(l, l));
+        if (n == 0.0) return mk(h.t[0], h.t[1], h.t[2]);
+        const double k = h.p[0] / n;
+        l = mk(l.x * k, l.y * k, l.z * k);
+        return to_world(h, l);
+    }
+    case D3D_ELLIPSE: {  // geometry.py:386-414
+        const double l0 = h.R[0] * d.x + h.R[3] * d.y + h.R[6] * d.z;
+        const double l1 = h.R[1] * d.x + h.R[4] * d.y + h.R[7] * d.z;
+        double q0 = h.p[0] * l0, q1 = h.p[1] * l1;
+        const double n = sqrt(q0 * q0 + q1 * q1);
+        if (n != 0.0) {
+            q0 = q0 / n;
+            q1 = q1 / n;
+        }
+        const double a0 = q0 * h.p[0], a1 = q1 * h.p[1];
+        return mk(h.t[0] + (a0 * h.R[0] + a1 * h.R[1]), h.t[1] + (a0 * h.R[3] + a1 * h.R[4]),
+                  h.t[2] + (a0 * h.R[6] + a1 * h.R[7]));
+    }
+    default:
+        return mk(NAN, NAN, NAN);
+    }
+}
+
+// Margin adds margin * norm_vector(d) (colliders.py:629-631).
+__device__ __forceinline__ v3 apply_margin(const Shape &h, v3 p, v3 d) {
+    if (h.margin != 0.0) p = p + norm_vector(d) * h.margin;
+    return p;
+}
+
+// Support of one collider per lane along d (per lane).  Hulls with more than
+// serial_max vertices are swept by the whole wavefront, one lane's hull at a
+// time.  Must be called by all 64 lanes (wave-uniform control flow); lanes
+// with want == false take part in the sweeps of the others.
+__device__ __forceinline__ v3 support_wave(const d3d_scene &s, const Shape &h, v3 d, bool want,
+                                           int serial_max) {
+    const bool coop = want && h.type == D3D_HULL && h.nv > serial_max;
+    v3 out = mk(0.0, 0.0, 0.0);
+    int idx = 0;
+    v3 l = mk(0.0, 0.0, 0.0);
+    if (want && h.type == D3D_HULL) l = to_local(h, d);
+    if (want && !coop) {
+        if (h.type == D3D_HULL) {
+            idx = serial_hull_argmax(s, h.voff, h.voff64, h.nv, l);
+        } else {
+            out = support_analytic(h, d);
+        }
+    }
+    unsigned long long m = __ballot(coop);
+    while (m) {
+        const int j = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int voff = readlane_i32(h.voff, j);
+        const int voff64 = readlane_i32(h.voff64, j);
+        const int nv = readlane_i32(h.nv, j);
+        const double L = readlane_f64(h.p[0], j);
+        const v3 lj = mk(readlane_f64(l.x, j), readlane_f64(l.y, j), readlane_f64(l.z, j));
+        const int best = coop_hull_argmax(s, voff, voff64, nv, L, lj);
+        if (lane_id() == j) idx = best;
+    }
+    if (want && h.type == D3D_HULL) out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));
+    if (want) out = apply_margin(h, out, d);
+    return out;
+}
+
+// ----------------------------------------------------- simplex routines
+// Straight restatements of gjk/_gjk_jolt.py; arrays are indexed with
+// compile-time constants only so the simplex stays in VGPRs.
+
+// get_barycentric_coordinates_line (_gjk_jolt.py:291-312)
+__device__ __forceinline__ void bary_line(v3 a, v3 b, double &u, double &v) {
+    const v3 ab = b - a;
+    const double den = dot(ab, ab);
+    if (den < kEpsSqr) {
+        if (dot(a, a) < dot(b, b)) {
+            u = 1.0;
+            v = 0.0;
+        } else {
+            u = 0.0;
+            v = 1.0;
+        }
+    } else {
+        v = -dot(a, ab) / den;
+        u = 1.0 - v;
+    }
+}
+
+// get_barycentric_coordinates_plane (_gjk_jolt.py:315-372)
+__device__ __forceinline__ void bary_plane(v3 a, v3 b, v3 c, double &u, double &v, double &w) {
+    const v3 v0 = b - a, v1 = c - a, v2 = c - b;
+    const double d00 = dot(v0, v0), d11 = dot(v1, v1), d22 = dot(v2, v2);
+    if (d00 <= d22) {
+        const double d01 = dot(v0, v1);
+        const double den = d00 * d11 - d01 * d01;
+        if (fabs(den) < kEps) {
+            if (d00 > d11) {
+                bary_line(a, b, u, v);
+                w = 0.0;
+            } else {
+                bary_line(a, c, u, w);
+                v = 0.0;
+            }
+        } else {
+            const double a0 = dot(a, v0), a1 = dot(a, v1);
+            v = (d01 * a1 - d11 * a0) / den;
+            w = (d01 * a0 - d00 * a1) / den;
+            u = 1.0 - v - w;
+        }
+    } else {
+        const double d12 = dot(v1, v2);
+        const double den = d11 * d22 - d12 * d12;
+        if (fabs(den) < kEps) {
+            if (d11 > d22) {
+                bary_line(a, c, u, w);
+                v = 0.0;
+            } else {
+                bary_line(b, c, v, w);
+                u = 0.0;
+            }
+        } else {
+            const double c1 = dot(c, v1), c2 = dot(c, v2);
+            u = (d22 * c1 - d12 * c2) / den;
+            v = (d11 * c2 - d12 * c1) / den;
+            w = 1.0 - u - v;
+        }
+    }
+}
+
+// get_barycentric_coordinates_tetrahedron (_gjk_jolt.py:375-390)
+__device__ __forceinline__ void bary_tetra(v3 a, v3 b, v3 c, v3 d, double &u, double &v,
+                                           double &w, double &x) {
+    const v3 vab = b - a, vac = c - a, vad = d - a;
+    const double va6 = -stp(b, d - b, c - b);
+    const double vb6 = -stp(a, vac, vad);
+    const double vc6 = -stp(a, vad, vab);
+    const double vd6 = -stp(a, vab, vac);
+    const double v6 = 1.0 / stp(vab, vac, vad);
+    u = va6 * v6;
+    v = vb6 * v6;
+    w = vc6 * v6;
+    x = vd6 * v6;
+}
+
+// closest_point_line (_gjk_jolt.py:393-412)
+__device__ __forceinline__ v3 cp_line(v3 a, v3 b, int &set) {
+    double u, v;
+    bary_line(a, b, u, v);
+    if (v <= 0.0) {
+        set = 1;
+        return a;
+    }
+    if (u <= 0.0) {
+        set = 2;
+        return b;
+    }
+    set = 3;
+    return a * u + b * v;
+}
+
+// closest_point_triangle (_gjk_jolt.py:415-523)
+__device__ __noinline__ v3 cp_triangle(v3 a, v3 b, v3 c, int &set) {
+    const v3 ab = b - a, ac = c - a, bc = c - b;
+    const v3 n = (dot(bc, bc) < dot(ac, ac)) ? cross(ab, bc) : cross(ab, ac);
+    const double nn = dot(n, n);
+    if (nn < kEpsSqr) {
+        int s, ns;
+        v3 cp = cp_line(a, b, s);
+        double best = dot(cp, cp);
+        v3 q = cp_line(a, c, ns);
+        double dd = dot(q, q);
+        if (dd < best) {
+            cp = q;
+            best = dd;
+            s = (ns & 1) + ((ns & 2) << 1);
+        }
+        q = cp_line(b, c, ns);
+        dd = dot(q, q);
+        if (dd < best) {
+            cp = q;
+            s = ns << 1;
+        }
+        set = s;
+        return cp;
+    }
+    const v3 ap = -a;
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        set = 1;
+        return a;
+    }
+    const v3 bp = -b;
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) {
+        set = 2;
+        return b;
+    }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && 0.0 <= d1 && d3 <= 0.0) {
+        const double v = d1 / (d1 - d3);
+        set = 3;
+        return a + ab * v;
+    }
+    const v3 cpt = -c;
+    const double d5 = dot(ab, cpt), d6 = dot(ac, cpt);
+    if (d6 >= 0.0 && d5 <= d6) {
+        set = 4;
+        return c;
+    }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && 0.0 <= d2 && d6 <= 0.0) {
+        const double w = d2 / (d2 - d6);
+        set = 5;
+        return a + ac * w;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    const double d4_d3 = d4 - d3, d5_d6 = d5 - d6;
+    if (va <= 0.0 && 0.0 <= d4_d3 && d5_d6 >= 0.0) {
+        const double w = d4_d3 / (d4_d3 + d5_d6);
+        set = 6;
+        return b + bc * w;
+    }
+    // n * (a + b + c).dot(n) / (3.0 * n_len_sq)
+    const double s = dot((a + b) + c, n);
+    const double den = 3.0 * nn;
+    set = 7;
+    return mk(n.x * s / den, n.y * s / den, n.z * s / den);
+}
+
+// closest_point_tetrahedron with origin_outside_of_tetrahedron_planes
+// (_gjk_jolt.py:526-631)
+__device__ __forceinline__ v3 cp_tetra(v3 a, v3 b, v3 c, v3 d, int &set) {
+    const v3 ab = b - a, ac = c - a, ad = d - a, bd = d - b, bc = c - b;
+    const v3 abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab), bdc = cross(bd, bc);
+    const double sp0 = dot(a, abc), sp1 = dot(a, acd), sp2 = dot(a, adb), sp3 = dot(b, bdc);
+    const double sd0 = dot(ad, abc), sd1 = dot(ab, acd), sd2 = dot(ac, adb), sd3 = -dot(ab, bdc);
+    bool o0, o1, o2, o3;
+    if (sd0 > 0.0 && sd1 > 0.0 && sd2 > 0.0 && sd3 > 0.0) {
+        o0 = sp0 >= -kEps;
+        o1 = sp1 >= -kEps;
+        o2 = sp2 >= -kEps;
+        o3 = sp3 >= -kEps;
+    } else if (sd0 < 0.0 && sd1 < 0.0 && sd2 < 0.0 && sd3 < 0.0) {
+        o0 = sp0 <= kEps;
+        o1 = sp1 <= kEps;
+        o2 = sp2 <= kEps;
+        o3 = sp3 <= kEps;
+    } else {
+        o0 = o1 = o2 = o3 = true;
+    }
+    int s = 15, ns;
+    v3 cp = mk(0.0, 0.0, 0.0);
+    double best = kMaxFloat;
+    if (o0) {
+        cp = cp_triangle(a, b, c, s);
+        best = dot(cp, cp);
+    }
+    if (o1) {
+        const v3 q = cp_triangle(a, c, d, ns);
+        const double dd = dot(q, q);
+        if (dd < best) {
+            best = dd;
+            cp = q;
+            s = (ns & 1) + ((ns & 6) << 1);
+        }
+    }
+    if (o2) {
+        const v3 q = cp_triangle(a, d, b, ns);
+        const double dd = dot(q, q);
+        if (dd < best) {
+            best = dd;
+            cp = q;
+            s = (ns & 1) + ((ns & 2) << 2) + ((ns & 4) >> 1);
+        }
+    }
+    if (o3) {
+        const v3 q = cp_triangle(b, d, c, ns);
+        const double dd = dot(q, q);
+        if (dd < best) {
+            cp = q;
+            s = ((ns & 1) << 1) + ((ns & 2) << 2) + (ns & 4);
+        }
+    }
+    set = s;
+    return cp;
+}
+
+// Simplex of up to four points with their supports on A and B.  Rows at
+// index >= n keep stale contents exactly as the reference's in-place
+// compaction leaves them.
+struct Simplex {
+    v3 Y[4], P[4], Q[4];
+    int n;
+};
+
+__device__ __forceinline__ void simplex_append(Simplex &S, v3 y, v3 p, v3 q) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (S.n == k) {
+            S.Y[k] = y;
+            S.P[k] = p;
+            S.Q[k] = q;
+        }
+    }
+    S.n += 1;
+}
+
+// update_simplex_ypq (_gjk_jolt.py:654-664); with_pq == false is
+// update_simplex_y (:643-651).
+template <bool with_pq>
+__device__ __forceinline__ void simplex_compact(Simplex &S, int set) {
+    Simplex T = S;
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool keep = (k < S.n) && ((set >> k) & 1);
+#pragma unroll
+        for (int j = 0; j <= k; ++j) {
+            if (keep && m == j) {
+                T.Y[j] = S.Y[k];
+                if (with_pq) {
+                    T.P[j] = S.P[k];
+                    T.Q[j] = S.Q[k];
+                }
+            }
+        }
+        m += keep ? 1 : 0;
+    }
+    T.n = m;
+    S = T;
+}
+
+// max_y_length_squared (_gjk_jolt.py:634-640)
+__device__ __forceinline__ double max_y_len_sq(const Simplex &S) {
+    double m = dot(S.Y[0], S.Y[0]);
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        if (k < S.n) {
+            const double l = dot(S.Y[k], S.Y[k]);
+            m = (l > m) ? l : m;
+        }
+    }
+    return m;
+}
+
+// get_closest_point_to_origin (_gjk_jolt.py:690-711)
+__device__ __forceinline__ bool closest_to_origin(const Simplex &S, double prev, v3 &v, double &vv,
+                                                  int &set) {
+    v3 p;
+    int s;
+    if (S.n == 1) {
+        s = 1;
+        p = S.Y[0];
+    } else if (S.n == 2) {
+        p = cp_line(S.Y[0], S.Y[1], s);
+    } else if (S.n == 3) {
+        p = cp_triangle(S.Y[0], S.Y[1], S.Y[2], s);
+    } else {
+        p = cp_tetra(S.Y[0], S.Y[1], S.Y[2], S.Y[3], s);
+    }
+    const double l = dot(p, p);
+    if (l < prev) {
+        v = p;
+        vv = l;
+        set = s;
+        return true;
+    }
+    return false;
+}
+
+// calculate_closest_points (_gjk_jolt.py:667-687)
+__device__ __forceinline__ void closest_points(const Simplex &S, v3 &a, v3 &b) {
+    if (S.n == 1) {
+        a = S.P[0];
+        b = S.Q[0];
+    } else if (S.n == 2) {
+        double u, v;
+        bary_line(S.Y[0], S.Y[1], u, v);
+        a = S.P[0] * u + S.P[1] * v;
+        b = S.Q[0] * u + S.Q[1] * v;
+    } else if (S.n == 3) {
+        double u, v, w;
+        bary_plane(S.Y[0], S.Y[1], S.Y[2], u, v, w);
+        a = (S.P[0] * u + S.P[1] * v) + S.P[2] * w;
+        b = (S.Q[0] * u + S.Q[1] * v) + S.Q[2] * w;
+    } else {
+        double u, v, w, x;
+        bary_tetra(S.Y[0], S.Y[1], S.Y[2], S.Y[3], u, v, w, x);
+        a = ((S.P[0] * u + S.P[1] * v) + S.P[2] * w) + S.P[3] * x;
+        b = ((S.Q[0] * u + S.Q[1] * v) + S.Q[2] * w) + S.Q[3] * x;
+    }
+}
+
+}  // namespace d3d

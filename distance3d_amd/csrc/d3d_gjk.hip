@@ -1,0 +1,503 @@
+// d3d_gjk.hip -- batched GJK distance / intersection, support points and
+// AABBs for gfx950.
+//
+// Kernel shape: one lane per pair, 64 pairs per wavefront.  The FP64 simplex
+// (Y, P, Q of up to four points) lives in VGPRs of its lane; support points of
+// analytic colliders and of small hulls are computed by the lane itself, while
+// large hulls are swept by the whole wavefront (coalesced FP32 SoA rows, see
+// coop_hull_argmax), one requesting lane at a time.  The loop runs until every
+// lane of the wavefront has terminated; finished lanes keep helping with the
+// sweeps of the others.
+#include "d3d_device.hpp"
+#include "d3d_runtime.hpp"
+
+namespace d3d {
+
+constexpr int kBlock = 256;
+constexpr int kStatusBadInput = 0x200;
+
+__device__ __forceinline__ Shape empty_shape() {
+    Shape h;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) h.R[j] = 0.0;
+    h.R[0] = h.R[4] = h.R[8] = 1.0;
+    h.t[0] = h.t[1] = h.t[2] = 0.0;
+    h.p[0] = h.p[1] = h.p[2] = 0.0;
+    h.margin = 0.0;
+    h.type = D3D_SPHERE;
+    h.nv = 0;
+    h.voff = 0;
+    h.voff64 = -1;
+    return h;
+}
+
+// Index and vertex-range checks so a malformed table cannot make a lane read
+// outside the pools.
+__device__ __forceinline__ bool load_checked(const d3d_scene &s, int k, Shape &h) {
+    if (k < 0 || (int64_t)k >= s.n_shapes) {
+        h = empty_shape();
+        return false;
+    }
+    h = load_shape(s, k);
+    if (h.type < D3D_HULL || h.type > D3D_ELLIPSE) {
+        h = empty_shape();
+        return false;
+    }
+    if (h.type == D3D_HULL) {
+        const bool ok32 = h.nv >= 1 && h.voff >= 0 && (int64_t)h.voff + h.nv <= s.n_vertices;
+        const bool ok64 =
+            h.voff64 < 0 || ((int64_t)h.voff64 + h.nv <= s.n_vertices64 && s.v64 != nullptr);
+        if (!ok32 || !ok64) {
+            h = empty_shape();
+            return false;
+        }
+    }
+    return true;
+}
+
+// gjk_distance_jolt (_gjk_jolt.py:138-221) and _distance_loop (:224-288).
+__global__ __launch_bounds__(kBlock) void gjk_distance_kernel(
+    d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, int64_t n,
+    d3d_gjk_params prm, double *__restrict__ dist, double *__restrict__ cpa,
+    double *__restrict__ cpb, int32_t *__restrict__ status, int32_t *__restrict__ iters,
+    double *__restrict__ simplex_out, int32_t *__restrict__ npts_out) {
+    const int64_t pid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pid < n;
+    Shape A = empty_shape(), B = empty_shape();
+    bool ok = false;
+    if (valid) {
+        const bool okA = load_checked(s, pa[pid], A);
+        const bool okB = load_checked(s, pb[pid], B);
+        ok = okA && okB;
+    }
+    Simplex S;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S.Y[k] = S.P[k] = S.Q[k] = mk(0.0, 0.0, 0.0);
+    S.n = 0;
+    v3 d = mk(1.0, 0.0, 0.0);
+    double vv = dot(d, d);
+    double prev = kMaxFloat;
+    const double tol2 = prm.tolerance * prm.tolerance;
+    const int serial_max = prm.hull_serial_max;
+    int state = -1, it = 0;
+    bool active = ok;
+
+    while (__ballot(active) != 0ull) {
+        if (active && it >= prm.max_iterations) {
+            state = D3D_ITER_CAP;
+            active = false;
+        }
+        const v3 p = support_wave(s, A, d, active, serial_max);
+        const v3 q = support_wave(s, B, -d, active, serial_max);
+        if (!active) continue;
+        ++it;
+        const v3 w = p - q;
+        const double dw = dot(d, w);
+        if (dw < 0.0 && dw * dw > vv * prm.max_distance_squared) {
+            state = D3D_CLIPPED;
+            active = false;
+            continue;
+        }
+        simplex_append(S, w, p, q);
+        v3 nvec;
+        double nvv;
+        int set;
+        if (closest_to_origin(S, prev, nvec, nvv, set)) {
+            d = nvec;
+            vv = nvv;
+        } else {
+            S.n -= 1;
+            set = (1 << S.n) - 1;
+        }
+        if (set == 15) {
+            vv = 0.0;
+            state = D3D_INTERSECT;
+            active = false;
+            continue;
+        }
+        simplex_compact<true>(S, set);
+        if (vv <= tol2 || vv <= kEps * max_y_len_sq(S)) {
+            vv = 0.0;
+            state = D3D_INTERSECT;
+            active = false;
+            continue;
+        }
+        d = -d;
+        if (prev - vv <= kEps * prev) {
+            state = D3D_SEPARATED;
+            active = false;
+            continue;
+        }
+        prev = vv;
+    }
+
+    if (!valid) return;
+    if (!ok) {
+        dist[pid] = NAN;
+        cpa[3 * pid] = cpa[3 * pid + 1] = cpa[3 * pid + 2] = NAN;
+        cpb[3 * pid] = cpb[3 * pid + 1] = cpb[3 * pid + 2] = NAN;
+        status[pid] = kStatusBadInput;
+        if (iters) iters[pid] = 0;
+        if (npts_out) npts_out[pid] = 0;
+        return;
+    }
+    if (iters) iters[pid] = it;
+    if (npts_out) npts_out[pid] = S.n;
+    if (simplex_out) {
+        double *o = simplex_out + 36 * pid;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            o[3 * k] = S.Y[k].x;
+            o[3 * k + 1] = S.Y[k].y;
+            o[3 * k + 2] = S.Y[k].z;
+            o[12 + 3 * k] = S.P[k].x;
+            o[13 + 3 * k] = S.P[k].y;
+            o[14 + 3 * k] = S.P[k].z;
+            o[24 + 3 * k] = S.Q[k].x;
+            o[25 + 3 * k] = S.Q[k].y;
+            o[26 + 3 * k] = S.Q[k].z;
+        }
+    }
+    if (state == D3D_CLIPPED) {
+        dist[pid] = kMaxFloat;
+        cpa[3 * pid] = cpa[3 * pid + 1] = cpa[3 * pid + 2] = NAN;
+        cpb[3 * pid] = cpb[3 * pid + 1] = cpb[3 * pid + 2] = NAN;
+        status[pid] = D3D_CLIPPED;
+        return;
+    }
+    v3 a, b;
+    closest_points(S, a, b);
+    int st = state;
+    if (!(fabs(dot(d, d) - vv) < prm.sanity_check)) st |= D3D_SANITY_FAILED;
+    const double dd = sqrt(vv);
+    if (dd < kEps) {
+        const v3 m = (a + b) * 0.5;
+        a = m;
+        b = m;
+    }
+    dist[pid] = dd;
+    cpa[3 * pid] = a.x;
+    cpa[3 * pid + 1] = a.y;
+    cpa[3 * pid + 2] = a.z;
+    cpb[3 * pid] = b.x;
+    cpb[3 * pid + 1] = b.y;
+    cpb[3 * pid + 2] = b.z;
+    status[pid] = st;
+}
+
+// gjk_intersection_jolt (_gjk_jolt.py:29-80) and _intersection_loop (:83-135).
+__global__ __launch_bounds__(kBlock) void gjk_intersection_kernel(
+    d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, int64_t n,
+    double tolerance, int max_iterations, int serial_max, uint8_t *__restrict__ hit,
+    int32_t *__restrict__ status) {
+    const int64_t pid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pid < n;
+    Shape A = empty_shape(), B = empty_shape();
+    bool ok = false;
+    if (valid) {
+        const bool okA = load_checked(s, pa[pid], A);
+        const bool okB = load_checked(s, pb[pid], B);
+        ok = okA && okB;
+    }
+    Simplex S;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) S.Y[k] = S.P[k] = S.Q[k] = mk(0.0, 0.0, 0.0);
+    S.n = 0;
+    v3 d = mk(1.0, 0.0, 0.0);
+    double prev = kMaxFloat;
+    const double tol2 = tolerance * tolerance;
+    int state = -1, it = 0;
+    bool active = ok;
+
+    while (__ballot(active) != 0ull) {
+        if (active && it >= max_iterations) {
+            state = D3D_ITER_CAP;
+            active = false;
+        }
+        const v3 p = support_wave(s, A, d, active, serial_max);
+        const v3 q = support_wave(s, B, -d, active, serial_max);
+        if (!active) continue;
+        ++it;
+        const v3 w = p - q;
+        if (dot(d, w) < -kEps) {
+            state = D3D_SEPARATED;
+            active = false;
+            continue;
+        }
+        simplex_append(S, w, w, w);
+        v3 v;
+        double vv;
+        int set;
+        if (!closest_to_origin(S, prev, v, vv, set)) {
+            state = D3D_SEPARATED;
+            active = false;
+            continue;
+        }
+        if (set == 15 || vv <= tol2 || vv <= kEps * max_y_len_sq(S)) {
+            state = D3D_INTERSECT;
+            active = false;
+            continue;
+        }
+        d = -v;
+        if (prev - vv <= kEps * prev) {
+            state = D3D_SEPARATED;
+            active = false;
+            continue;
+        }
+        prev = vv;
+        simplex_compact<false>(S, set);
+    }
+    if (!valid) return;
+    if (!ok) {
+        hit[pid] = 0;
+        if (status) status[pid] = kStatusBadInput;
+        return;
+    }
+    hit[pid] = state == D3D_INTERSECT ? 1 : 0;
+    if (status) status[pid] = state;
+}
+
+// collider.support_function(d) for a batch of (collider, direction) queries.
+__global__ __launch_bounds__(kBlock) void support_kernel(d3d_scene s,
+                                                         const int32_t *__restrict__ shape,
+                                                         const double *__restrict__ dirs, int64_t n,
+                                                         int serial_max, double *__restrict__ out) {
+    const int64_t pid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pid < n;
+    Shape h = empty_shape();
+    bool ok = false;
+    v3 d = mk(0.0, 0.0, 0.0);
+    if (valid) {
+        ok = load_checked(s, shape[pid], h);
+        d = mk(dirs[3 * pid], dirs[3 * pid + 1], dirs[3 * pid + 2]);
+    }
+    const v3 p = support_wave(s, h, d, ok, serial_max);
+    if (!valid) return;
+    out[3 * pid] = ok ? p.x : NAN;
+    out[3 * pid + 1] = ok ? p.y : NAN;
+    out[3 * pid + 2] = ok ? p.z : NAN;
+}
+
+// collider.aabb() (containment.py:6-229), lane per collider.
+__global__ __launch_bounds__(kBlock) void aabb_kernel(d3d_scene s, const int32_t *__restrict__ shape,
+                                                      int64_t n, double *__restrict__ out) {
+    const int64_t pid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= n) return;
+    Shape h;
+    double mn[3], mx[3];
+    if (!load_checked(s, shape[pid], h)) {
+#pragma unroll
+        for (int a = 0; a < 6; ++a) out[6 * pid + a] = NAN;
+        return;
+    }
+    const double *R = h.R, *t = h.t;
+    switch (h.type) {
+    case D3D_HULL: {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = INFINITY;
+            mx[a] = -INFINITY;
+        }
+        for (int j = 0; j < h.nv; ++j) {
+            const v3 w = to_world(h, hull_vertex(s, h.voff, h.voff64, j));
+            mn[0] = w.x < mn[0] ? w.x : mn[0];
+            mx[0] = w.x > mx[0] ? w.x : mx[0];
+            mn[1] = w.y < mn[1] ? w.y : mn[1];
+            mx[1] = w.y > mx[1] ? w.y : mx[1];
+            mn[2] = w.z < mn[2] ? w.z : mn[2];
+            mx[2] = w.z > mx[2] ? w.z : mx[2];
+        }
+        break;
+    }
+    case D3D_BOX: {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = INFINITY;
+            mx[a] = -INFINITY;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const double l0 = (((c >> 2) & 1) ? 0.5 : -0.5) * h.p[0];
+            const double l1 = (((c >> 1) & 1) ? 0.5 : -0.5) * h.p[1];
+            const double l2 = ((c & 1) ? 0.5 : -0.5) * h.p[2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double v = t[a] + (l0 * R[3 * a] + l1 * R[3 * a + 1] + l2 * R[3 * a + 2]);
+                mn[a] = v < mn[a] ? v : mn[a];
+                mx[a] = v > mx[a] ? v : mx[a];
+            }
+        }
+        break;
+    }
+    case D3D_SPHERE:
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = t[a] - h.p[0];
+            mx[a] = t[a] + h.p[0];
+        }
+        break;
+    case D3D_CAPSULE:
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double e = 0.5 * h.p[1] * fabs(R[3 * a + 2]) + h.p[0];
+            mn[a] = t[a] - e;
+            mx[a] = t[a] + e;
+        }
+        break;
+    case D3D_CYLINDER:
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double ax = R[3 * a + 2];
+            const double e = 0.5 * h.p[1] * fabs(ax) + h.p[0] * sqrt(1.0 - ax * ax);
+            mn[a] = t[a] - e;
+            mx[a] = t[a] + e;
+        }
+        break;
+    case D3D_ELLIPSOID: {
+        double E[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) E[i][j] = R[3 * i + j] * h.p[j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double cn = sqrt(E[0][j] * E[0][j] + E[1][j] * E[1][j] + E[2][j] * E[2][j]);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) E[i][j] = E[i][j] / cn;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) E[i][j] = E[i][j] * h.p[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double m = -INFINITY;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double v = R[3 * i] * E[k][0] + R[3 * i + 1] * E[k][1] + R[3 * i + 2] * E[k][2];
+                m = v > m ? v : m;
+            }
+            mn[k] = t[k] - m;
+            mx[k] = t[k] + m;
+        }
+        break;
+    }
+    case D3D_CONE:
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double ht = h.p[1], r = h.p[0];
+            const double pa_ = t[a], pb_ = t[a] + ht * R[3 * a + 2];
+            const double ad = pb_ - pa_;
+            const double e = sqrt(1.0 - ad * ad / (ht * ht));
+            const double lo = pa_ - e * r, hi = pa_ + e * r;
+            mn[a] = lo < pb_ ? lo : pb_;
+            mx[a] = hi > pb_ ? hi : pb_;
+        }
+        break;
+    case D3D_DISK:
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double nr = R[3 * a + 2];
+            const double e = h.p[0] * sqrt(1.0 - nr * nr);
+            mn[a] = t[a] - e;
+            mx[a] = t[a] + e;
+        }
+        break;
+    default:  // D3D_ELLIPSE
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double u = h.p[0] * R[3 * a], v = h.p[1] * R[3 * a + 1];
+            const double e = sqrt(u * u + v * v);
+            mn[a] = t[a] - e;
+            mx[a] = t[a] + e;
+        }
+        break;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (h.margin != 0.0) {
+            mn[a] = mn[a] - h.margin;
+            mx[a] = mx[a] + h.margin;
+        }
+        out[6 * pid + 2 * a] = mn[a];
+        out[6 * pid + 2 * a + 1] = mx[a];
+    }
+}
+
+}  // namespace d3d
+
+using namespace d3d;
+
+static d3d_gjk_params default_params() {
+    d3d_gjk_params p;
+    p.tolerance = 1e-10;
+    p.max_distance_squared = 100000.0;
+    p.sanity_check = 1e-8;
+    p.max_iterations = 128;
+    p.hull_serial_max = 16;
+    return p;
+}
+
+extern "C" int d3d_support_batch(const d3d_scene *scene, const int32_t *shape, const double *dirs,
+                                 int64_t n, double *out, void *stream) {
+    clear_error();
+    if (int rc = check_scene(scene)) return rc;
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(shape && dirs && out, "NULL buffer");
+    hipLaunchKernelGGL(support_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, *scene, shape, dirs, n, default_params().hull_serial_max,
+                       out);
+    D3D_CHECK_LAUNCH("support_kernel");
+    return D3D_OK;
+}
+
+extern "C" int d3d_aabb_batch(const d3d_scene *scene, const int32_t *shape, int64_t n, double *out,
+                              void *stream) {
+    clear_error();
+    if (int rc = check_scene(scene)) return rc;
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(shape && out, "NULL buffer");
+    hipLaunchKernelGGL(aabb_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
+                       *scene, shape, n, out);
+    D3D_CHECK_LAUNCH("aabb_kernel");
+    return D3D_OK;
+}
+
+extern "C" int d3d_gjk_distance_batch(const d3d_scene *scene, const int32_t *pair_a,
+                                      const int32_t *pair_b, int64_t n,
+                                      const d3d_gjk_params *params, double *dist, double *cp_a,
+                                      double *cp_b, int32_t *status, int32_t *iterations,
+                                      double *simplex, int32_t *n_points, void *stream) {
+    clear_error();
+    if (int rc = check_scene(scene)) return rc;
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(pair_a && pair_b && dist && cp_a && cp_b && status, "NULL buffer");
+    const d3d_gjk_params prm = params ? *params : default_params();
+    D3D_CHECK_ARG(prm.max_iterations > 0, "max_iterations must be > 0");
+    D3D_CHECK_ARG(prm.hull_serial_max >= 0, "hull_serial_max must be >= 0");
+    hipLaunchKernelGGL(gjk_distance_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, *scene, pair_a, pair_b, n, prm, dist, cp_a, cp_b, status,
+                       iterations, simplex, n_points);
+    D3D_CHECK_LAUNCH("gjk_distance_kernel");
+    return D3D_OK;
+}
+
+extern "C" int d3d_gjk_intersection_batch(const d3d_scene *scene, const int32_t *pair_a,
+                                          const int32_t *pair_b, int64_t n, double tolerance,
+                                          int32_t max_iterations, uint8_t *hit, int32_t *status,
+                                          void *stream) {
+    clear_error();
+    if (int rc = check_scene(scene)) return rc;
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(pair_a && pair_b && hit, "NULL buffer");
+    D3D_CHECK_ARG(max_iterations > 0, "max_iterations must be > 0");
+    hipLaunchKernelGGL(gjk_intersection_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, *scene, pair_a, pair_b, n, tolerance, max_iterations,
+                       default_params().hull_serial_max, hit, status);
+    D3D_CHECK_LAUNCH("gjk_intersection_kernel");
+    return D3D_OK;
+}

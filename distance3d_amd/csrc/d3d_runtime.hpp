@@ -1,0 +1,65 @@
+// d3d_runtime.hpp -- error plumbing shared by the C-ABI entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "d3d.h"
+
+namespace d3d {
+
+void set_error(const std::string &msg);
+void clear_error();
+
+#define D3D_CHECK_ARG(cond, msg)            \
+    do {                                    \
+        if (!(cond)) {                      \
+            ::d3d::set_error(msg);          \
+            return D3D_E_INVALID;           \
+        }                                   \
+    } while (0)
+
+#define D3D_CHECK_HIP(expr)                                                          \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            ::d3d::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+            return D3D_E_HIP;                                                        \
+        }                                                                            \
+    } while (0)
+
+// Checks a kernel launch (launch errors only; execution is asynchronous).
+#define D3D_CHECK_LAUNCH(name)                                                       \
+    do {                                                                             \
+        hipError_t e_ = hipGetLastError();                                           \
+        if (e_ != hipSuccess) {                                                      \
+            ::d3d::set_error(std::string(name) + " launch: " + hipGetErrorString(e_)); \
+            return D3D_E_HIP;                                                        \
+        }                                                                            \
+    } while (0)
+
+inline int check_scene(const d3d_scene *s) {
+    if (!s) {
+        set_error("scene is NULL");
+        return D3D_E_INVALID;
+    }
+    if (!s->shape_f64 || !s->shape_i32 || s->n_shapes <= 0) {
+        set_error("scene has no collider table");
+        return D3D_E_INVALID;
+    }
+    if (s->n_vertices > 0 && (!s->vx || !s->vy || !s->vz)) {
+        set_error("scene has vertices but a NULL FP32 pool pointer");
+        return D3D_E_INVALID;
+    }
+    if (s->n_vertices64 > 0 && !s->v64) {
+        set_error("scene has FP64 vertices but a NULL v64 pointer");
+        return D3D_E_INVALID;
+    }
+    return D3D_OK;
+}
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace d3d

@@ -1,0 +1,113 @@
+"""Batched entry points over a device-resident scene (torch tensors in, torch
+tensors out).  Each function is one call into libd3d_hip on the current
+stream; none of them synchronises."""
+import numpy as np
+
+from . import _native
+from ._native import check, lib, ptr, stream_ptr
+
+
+def _dev_i32(x, device):
+    import torch
+    return torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x,
+                           dtype=torch.int32).to(device).contiguous()
+
+
+def _dev_f64(x, device):
+    import torch
+    return torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x,
+                           dtype=torch.float64).to(device).contiguous()
+
+
+def support_batch(scene, shape_ids, dirs, stream=None):
+    """Support points of ``scene`` colliders ``shape_ids`` along ``dirs``
+    (n, 3) -> (n, 3) float64 tensor."""
+    import torch
+    ids = _dev_i32(shape_ids, scene.device)
+    d = _dev_f64(dirs, scene.device).reshape(-1, 3)
+    n = ids.numel()
+    if d.shape[0] != n:
+        raise ValueError("shape_ids and dirs differ in length")
+    out = torch.empty((n, 3), dtype=torch.float64, device=scene.device)
+    check(lib().d3d_support_batch(scene.c, ptr(ids), ptr(d), n, ptr(out), stream_ptr(stream)),
+          "d3d_support_batch")
+    return out
+
+
+def aabb_batch(scene, shape_ids, stream=None):
+    """AABBs (n, 3, 2) of ``scene`` colliders ``shape_ids``."""
+    import torch
+    ids = _dev_i32(shape_ids, scene.device)
+    n = ids.numel()
+    out = torch.empty((n, 3, 2), dtype=torch.float64, device=scene.device)
+    check(lib().d3d_aabb_batch(scene.c, ptr(ids), n, ptr(out), stream_ptr(stream)),
+          "d3d_aabb_batch")
+    return out
+
+
+class GjkResult:
+    """Outputs of :func:`gjk_distance_batch` (device tensors)."""
+
+    def __init__(self, dist, cp_a, cp_b, status, iterations, simplex, n_points):
+        self.dist = dist
+        self.cp_a = cp_a
+        self.cp_b = cp_b
+        self.status = status
+        self.iterations = iterations
+        self.simplex = simplex
+        self.n_points = n_points
+
+    @property
+    def intersecting(self):
+        """dist == 0.0, the reference's intersection criterion."""
+        return self.dist == 0.0
+
+
+def gjk_distance_batch(scene, pair_a, pair_b, tolerance=1e-10, max_distance_squared=100000.0,
+                       sanity_check=1e-8, max_iterations=128, hull_serial_max=16,
+                       want_simplex=False, want_iterations=True, out=None, stream=None):
+    """GJK distance over pairs (gjk.gjk semantics per pair).
+
+    ``out`` may hold preallocated tensors (dist, cp_a, cp_b, status,
+    iterations, simplex, n_points) to avoid allocation in a loop."""
+    import torch
+    a = _dev_i32(pair_a, scene.device)
+    b = _dev_i32(pair_b, scene.device)
+    n = a.numel()
+    if b.numel() != n:
+        raise ValueError("pair_a and pair_b differ in length")
+    dev = scene.device
+    if out is None:
+        out = GjkResult(
+            torch.empty(n, dtype=torch.float64, device=dev),
+            torch.empty((n, 3), dtype=torch.float64, device=dev),
+            torch.empty((n, 3), dtype=torch.float64, device=dev),
+            torch.empty(n, dtype=torch.int32, device=dev),
+            torch.empty(n, dtype=torch.int32, device=dev) if want_iterations else None,
+            torch.empty((n, 3, 4, 3), dtype=torch.float64, device=dev) if want_simplex else None,
+            torch.empty(n, dtype=torch.int32, device=dev) if want_simplex else None)
+    prm = _native.gjk_params(tolerance, max_distance_squared, sanity_check, max_iterations,
+                             hull_serial_max)
+    import ctypes
+    check(lib().d3d_gjk_distance_batch(
+        scene.c, ptr(a), ptr(b), n, ctypes.byref(prm), ptr(out.dist), ptr(out.cp_a),
+        ptr(out.cp_b), ptr(out.status), ptr(out.iterations), ptr(out.simplex),
+        ptr(out.n_points), stream_ptr(stream)), "d3d_gjk_distance_batch")
+    return out
+
+
+def gjk_intersection_batch(scene, pair_a, pair_b, tolerance=1e-10, max_iterations=128,
+                           want_status=False, stream=None):
+    """GJK intersection test over pairs -> (hit uint8 tensor, status or None)."""
+    import torch
+    a = _dev_i32(pair_a, scene.device)
+    b = _dev_i32(pair_b, scene.device)
+    n = a.numel()
+    if b.numel() != n:
+        raise ValueError("pair_a and pair_b differ in length")
+    hit = torch.empty(n, dtype=torch.uint8, device=scene.device)
+    status = torch.empty(n, dtype=torch.int32, device=scene.device) if want_status else None
+    check(lib().d3d_gjk_intersection_batch(
+        scene.c, ptr(a), ptr(b), n, float(tolerance), int(max_iterations), ptr(hit),
+        ptr(status), stream_ptr(stream)), "d3d_gjk_intersection_batch")
+    return hit, status

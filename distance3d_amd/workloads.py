@@ -1,0 +1,193 @@
+"""Synthetic collision workloads, packed straight into scene arrays.
+
+Every generator uses ``numpy.random.default_rng(seed)``; rotations are
+uniform quaternions (normalised N(0, I4)).  Hull vertices are drawn in FP64,
+rounded once to FP32 in the collider's local frame, and that FP32 set with an
+FP64 pose is the geometry both the GPU and the FP64 checker see.
+
+* ``hull_pairs``   -- 4096 pairs of 32-vertex hulls (unit-sphere points scaled
+  by U(0.2, 1), centres U[-2, 2]^3), seed 0.
+* ``mixed_pairs``  -- 1,048,576 pairs, each side uniform over {sphere, capsule,
+  cylinder, box, ellipsoid, hull with log-uniform 8..4096 vertices on a
+  perturbed unit sphere}, centres U[-3, 3]^3, seed 1.
+* ``epa_pairs``    -- 262,144 overlapping hull pairs, 64..256 vertices on
+  ellipsoid surfaces (radii U(0.3, 1)), projections along a random axis
+  overlapping by U(0.01, 0.5), seed 2.
+"""
+import numpy as np
+
+from . import _native
+from .scene import F64_COLS, I32_COLS, HostScene
+
+_ULP_MARGIN = 1.0 + 2.0 ** -20
+
+
+def uniform_rotations(rng, n):
+    """(n, 3, 3) rotation matrices from uniform unit quaternions."""
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def _bounds_per_hull(v32, counts):
+    """max(|x|+|y|+|z|) per hull over a concatenated (N,3) FP32 array."""
+    s = np.abs(v32.astype(np.float64)).sum(axis=1)
+    starts = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    return np.maximum.reduceat(s, starts) * _ULP_MARGIN
+
+
+class Workload:
+    """A packed scene plus the pair list to evaluate on it."""
+
+    def __init__(self, name, scene, pair_a, pair_b, meta=None):
+        self.name = name
+        self.scene = scene
+        self.pair_a = np.ascontiguousarray(pair_a, dtype=np.int32)
+        self.pair_b = np.ascontiguousarray(pair_b, dtype=np.int32)
+        self.meta = meta or {}
+
+    @property
+    def n_pairs(self):
+        return len(self.pair_a)
+
+    def hull_bytes_per_pair(self):
+        """12 * (n_A + n_B): FP32 hull bytes a pair reads once."""
+        i = self.scene.shape_i32
+        nv = np.where(i[:, 0] == _native.HULL, i[:, 1], 0).astype(np.int64)
+        return 12 * (nv[self.pair_a] + nv[self.pair_b])
+
+    def subset(self, idx):
+        idx = np.asarray(idx)
+        return Workload(self.name + "[subset]", self.scene, self.pair_a[idx], self.pair_b[idx],
+                        dict(self.meta))
+
+
+def _scene_from(types, R, t, params, hull_counts, v32):
+    n = len(types)
+    f = np.zeros((n, F64_COLS))
+    f[:, 0:9] = R.reshape(n, 9)
+    f[:, 9:12] = t
+    f[:, 12:15] = params
+    i = np.zeros((n, I32_COLS), dtype=np.int32)
+    i[:, 0] = types
+    i[:, 3] = -1
+    hull = types == _native.HULL
+    counts = hull_counts[hull].astype(np.int64)
+    offs = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    i[hull, 1] = counts
+    i[hull, 2] = offs
+    if len(v32):
+        f[hull, 12] = _bounds_per_hull(v32, counts)
+        vx, vy, vz = (np.ascontiguousarray(v32[:, k]) for k in range(3))
+    else:
+        vx = vy = vz = np.zeros(0, dtype=np.float32)
+    return HostScene(f, i, vx, vy, vz)
+
+
+def _sphere_points(rng, total, chunk=1 << 24):
+    """(total, 3) FP64 unit vectors, generated in chunks."""
+    out = np.empty((total, 3))
+    for s in range(0, total, chunk):
+        e = min(total, s + chunk)
+        g = rng.standard_normal((e - s, 3))
+        g /= np.linalg.norm(g, axis=1, keepdims=True)
+        out[s:e] = g
+    return out
+
+
+def hull_pairs(seed=0, n_pairs=4096, n_vertices=32):
+    """4096 pairs of 32-vertex hulls (the CPU-runnable configuration)."""
+    rng = np.random.default_rng(seed)
+    n = 2 * n_pairs
+    v = _sphere_points(rng, n * n_vertices).reshape(n, n_vertices, 3)
+    s = rng.uniform(0.2, 1.0, n)
+    R = uniform_rotations(rng, n)
+    c = rng.uniform(-2.0, 2.0, (n, 3))
+    v32 = (v * s[:, None, None]).astype(np.float32).reshape(-1, 3)
+    types = np.full(n, _native.HULL, dtype=np.int32)
+    counts = np.full(n, n_vertices, dtype=np.int64)
+    scene = _scene_from(types, R, c, np.zeros((n, 3)), counts, v32)
+    return Workload("hull_pairs", scene, np.arange(0, n, 2), np.arange(1, n, 2),
+                    {"seed": seed, "n_pairs": n_pairs, "n_vertices": n_vertices})
+
+
+MIXED_TYPES = np.array([_native.SPHERE, _native.CAPSULE, _native.CYLINDER, _native.BOX,
+                        _native.ELLIPSOID, _native.HULL], dtype=np.int32)
+
+
+def mixed_pairs(seed=1, n_pairs=1 << 20, min_vertices=8, max_vertices=4096):
+    """Mixed-primitive pairs; every pair has its own two colliders."""
+    rng = np.random.default_rng(seed)
+    n = 2 * n_pairs
+    types = MIXED_TYPES[rng.integers(0, len(MIXED_TYPES), n)]
+    R = uniform_rotations(rng, n)
+    t = rng.uniform(-3.0, 3.0, (n, 3))
+    params = np.zeros((n, 3))
+    u = rng.uniform(size=(n, 3))
+    sph = types == _native.SPHERE
+    params[sph, 0] = 0.1 + 0.9 * u[sph, 0]
+    R[sph] = np.eye(3)
+    for ty in (_native.CAPSULE, _native.CYLINDER):
+        m = types == ty
+        params[m, 0] = 0.05 + 0.45 * u[m, 0]
+        params[m, 1] = 0.2 + 1.8 * u[m, 1]
+    box = types == _native.BOX
+    params[box] = 0.1 + 1.9 * u[box]
+    ell = types == _native.ELLIPSOID
+    params[ell] = 0.1 + 0.9 * u[ell]
+    hull = types == _native.HULL
+    counts = np.zeros(n, dtype=np.int64)
+    nh = int(hull.sum())
+    counts[hull] = np.rint(np.exp(rng.uniform(np.log(min_vertices), np.log(max_vertices), nh)))
+    total = int(counts.sum())
+    v = _sphere_points(rng, total)
+    radius = 1.0 + rng.uniform(-0.1, 0.1, total)
+    v32 = (v * radius[:, None]).astype(np.float32)
+    del v, radius
+    scene = _scene_from(types, R, t, params, counts, v32)
+    return Workload("mixed_pairs", scene, np.arange(0, n, 2), np.arange(1, n, 2),
+                    {"seed": seed, "n_pairs": n_pairs})
+
+
+def epa_pairs(seed=2, n_pairs=262144, min_vertices=64, max_vertices=256):
+    """Overlapping hull pairs for penetration depth."""
+    rng = np.random.default_rng(seed)
+    n = 2 * n_pairs
+    counts = rng.integers(min_vertices, max_vertices + 1, n).astype(np.int64)
+    radii = rng.uniform(0.3, 1.0, (n, 3))
+    total = int(counts.sum())
+    owner = np.repeat(np.arange(n), counts)
+    v = _sphere_points(rng, total) * radii[owner]
+    v32 = v.astype(np.float32)
+    del v
+    R = uniform_rotations(rng, n)
+    axis = _sphere_points(rng, n_pairs)
+    delta = rng.uniform(0.01, 0.5, n_pairs)
+    cA = rng.uniform(-3.0, 3.0, (n_pairs, 3))
+    # support values h_A(u) and h_B(-u) of the posed FP32 hulls (centred at 0)
+    local_dir = np.empty((n, 3))
+    local_dir[0::2] = np.einsum("kji,kj->ki", R[0::2], axis)
+    local_dir[1::2] = np.einsum("kji,kj->ki", R[1::2], -axis)
+    proj = np.einsum("ij,ij->i", v32.astype(np.float64), local_dir[owner])
+    starts = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    h = np.maximum.reduceat(proj, starts)
+    gap = h[0::2] + h[1::2] - delta
+    cB = cA + axis * gap[:, None]
+    t = np.empty((n, 3))
+    t[0::2] = cA
+    t[1::2] = cB
+    types = np.full(n, _native.HULL, dtype=np.int32)
+    scene = _scene_from(types, R, t, np.zeros((n, 3)), counts, v32)
+    return Workload("epa_pairs", scene, np.arange(0, n, 2), np.arange(1, n, 2),
+                    {"seed": seed, "n_pairs": n_pairs, "delta": delta, "axis": axis})

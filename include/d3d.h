@@ -1,0 +1,153 @@
+/*
+ * d3d.h -- C ABI of libd3d_hip.so, the MI355X (gfx950) collision hot path.
+ *
+ * Every entry point:
+ *   - returns 0 on success and a nonzero D3D_E_* code on failure (it never
+ *     throws across the ABI); d3d_last_error() then describes the failure
+ *     (thread-local string);
+ *   - takes caller-owned DEVICE pointers for every array argument (for
+ *     example torch.Tensor.data_ptr() of a tensor on the GPU) unless the
+ *     argument is documented as host memory;
+ *   - enqueues work on `stream` (a hipStream_t passed as void*; NULL is the
+ *     default stream) and does not synchronise unless documented.
+ *
+ * The reference (distance3d 0.9.0) is a Python library with no FFI; each
+ * entry point below replaces the reference function cited beside it, batched
+ * over many pairs.  The Python package distance3d_amd binds these with ctypes
+ * and keeps the reference's call surface (gjk.gjk, epa.epa, colliders.*,
+ * aabb_tree.AabbTree).
+ */
+#ifndef D3D_H
+#define D3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define D3D_ABI_VERSION 1
+
+/* error codes */
+#define D3D_OK 0
+#define D3D_E_INVALID 1  /* bad argument (null pointer, size, shape index) */
+#define D3D_E_HIP 2      /* a HIP runtime call failed */
+#define D3D_E_CAPACITY 3 /* an output buffer was too small */
+
+/* collider types (shape_i32[k*4 + 0]) */
+#define D3D_HULL 0      /* ConvexHullVertices / Box-free vertex sets, MeshGraph */
+#define D3D_SPHERE 1    /* Sphere */
+#define D3D_CAPSULE 2   /* Capsule */
+#define D3D_ELLIPSOID 3 /* Ellipsoid */
+#define D3D_CYLINDER 4  /* Cylinder */
+#define D3D_BOX 5       /* Box */
+#define D3D_CONE 6      /* Cone */
+#define D3D_DISK 7      /* Disk */
+#define D3D_ELLIPSE 8   /* Ellipse */
+
+/* GJK status (low byte = GjkState of gjk/_gjk_jolt.py:22-26) */
+#define D3D_SEPARATED 0
+#define D3D_INTERSECT 1
+#define D3D_CLIPPED 3
+#define D3D_ITER_CAP 4
+#define D3D_SANITY_FAILED 0x100 /* bit set when |d.d - v_len_sq| >= sanity_check */
+
+/*
+ * A scene: the collider table and the vertex pools, resident in HBM.
+ *   shape_f64[k*16 + 0..8]  R = collider2origin[:3,:3], row major
+ *   shape_f64[k*16 + 9..11] t = collider2origin[:3,3]
+ *   shape_f64[k*16 + 12..14] type parameters:
+ *       HULL: p0 = max_i(|x_i|+|y_i|+|z_i|) over the local vertices
+ *       SPHERE: radius; CAPSULE: radius, height; CYLINDER: radius, length;
+ *       CONE: radius, height; DISK: radius (R columns = plane basis, normal);
+ *       ELLIPSE: radius0, radius1 (R columns 0,1 = axes); BOX: size[3];
+ *       ELLIPSOID: radii[3]
+ *   shape_f64[k*16 + 15]    margin (Margin collider), 0 otherwise
+ *   shape_i32[k*4 + 0..3]   type, n_vertices, offset into vx/vy/vz,
+ *                           offset into v64 (-1 when the FP32 values are exact)
+ *   vx, vy, vz              FP32 vertex pool, structure of arrays, local frame
+ *   v64                     FP64 vertex pool, (n, 3) row major, for hulls
+ *                           whose vertices are not exactly representable in
+ *                           FP32 (may be NULL when no shape uses it)
+ */
+typedef struct d3d_scene {
+    const double *shape_f64;
+    const int32_t *shape_i32;
+    const float *vx;
+    const float *vy;
+    const float *vz;
+    const double *v64;
+    int64_t n_shapes;
+    int64_t n_vertices;
+    int64_t n_vertices64;
+} d3d_scene;
+
+/* gjk_distance_jolt keyword arguments (gjk/_gjk_jolt.py:138-139) plus the
+ * iteration cap the reference loop lacks. */
+typedef struct d3d_gjk_params {
+    double tolerance;            /* 1e-10 */
+    double max_distance_squared; /* 1e5 */
+    double sanity_check;         /* 1e-8 */
+    int32_t max_iterations;      /* 128 */
+    int32_t hull_serial_max;     /* hulls with <= this many vertices are swept
+                                    by their own lane; larger hulls by the
+                                    whole wavefront (default 16) */
+} d3d_gjk_params;
+
+/* Library version / device setup. */
+int d3d_abi_version(void);
+const char *d3d_last_error(void);
+/* Selects `device` for the calling thread and checks it is a gfx950 GPU. */
+int d3d_init(int device);
+
+/*
+ * Support points, collider.support_function(d) (colliders.py:131-132,
+ * 272-274, 326-328, 374-376, 426-428, 479-481, 533-535, 590-592, 629-631;
+ * geometry.py:160-454).
+ *   shape (n) int32, dirs (n,3) f64 world directions -> out (n,3) f64.
+ */
+int d3d_support_batch(const d3d_scene *scene, const int32_t *shape, const double *dirs,
+                      int64_t n, double *out, void *stream);
+
+/*
+ * Axis-aligned bounding boxes, collider.aabb() (colliders.py:140-141,
+ * 180-181, 281-282, 335-337, 383-384, 435-437, 489-490, 543-544, 599-600,
+ * 639-643; containment.py:6-229).
+ *   shape (n) int32 -> out (n,3,2) f64 [axis][min,max].
+ */
+int d3d_aabb_batch(const d3d_scene *scene, const int32_t *shape, int64_t n, double *out,
+                   void *stream);
+
+/*
+ * Batched GJK distance, gjk.gjk / gjk_distance_jolt (gjk/_gjk_jolt.py:138-221).
+ *   pair_a, pair_b (n) int32 shape indices.
+ *   dist (n) f64: distance, DBL_MAX when clipped (the reference's MAX_FLOAT).
+ *   cp_a, cp_b (n,3) f64: closest points (NaN when clipped).
+ *   status (n) int32: D3D_SEPARATED / INTERSECT / CLIPPED / ITER_CAP, with
+ *       D3D_SANITY_FAILED or-ed in when the reference's sanity assert
+ *       (gjk/_gjk_jolt.py:215-216) would fire.
+ *   iterations (n) int32, optional (NULL).
+ *   simplex (n,3,4,3) f64, optional: final Y, P, Q rows (the reference's
+ *       returned simplex is the Y block); n_points (n) int32, optional.
+ */
+int d3d_gjk_distance_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
+                           int64_t n, const d3d_gjk_params *params, double *dist, double *cp_a,
+                           double *cp_b, int32_t *status, int32_t *iterations, double *simplex,
+                           int32_t *n_points, void *stream);
+
+/*
+ * Batched GJK intersection, gjk.gjk_intersection / gjk_intersection_jolt
+ * (gjk/_gjk_jolt.py:29-135).
+ *   hit (n) uint8: 1 intersecting, 0 separated.
+ *   status (n) int32, optional: D3D_SEPARATED / INTERSECT / ITER_CAP.
+ */
+int d3d_gjk_intersection_batch(const d3d_scene *scene, const int32_t *pair_a,
+                               const int32_t *pair_b, int64_t n, double tolerance,
+                               int32_t max_iterations, uint8_t *hit, int32_t *status,
+                               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* D3D_H */

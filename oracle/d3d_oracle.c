@@ -1,0 +1,1065 @@
+/*
+ * d3d_oracle.c -- FP64 CPU restatement of distance3d's collision hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (distance3d_amd/) never links or calls it.
+ *
+ * What is restated (reference file:line in each function's comment):
+ *   - support functions of every collider       geometry.py:138-454, colliders.py:109-646
+ *   - Jolt GJK distance + closest points        gjk/_gjk_jolt.py:138-288, 291-711
+ *   - Jolt GJK intersection                     gjk/_gjk_jolt.py:29-135
+ *   - EPA penetration depth                     epa.py:9-202 (robust variant, see epa_pair)
+ *   - AABBs of every collider                   containment.py:6-229, colliders.py:140-646
+ *   - closed-interval AABB overlap / all pairs  aabb_tree.py:465-527
+ *
+ * Arithmetic: every expression is written left to right in the order the
+ * reference writes it and the file is compiled with -ffp-contract=off, so each
+ * product and sum is rounded once, as numpy does elementwise.
+ *
+ * Scene format (shared with the product's packed scene):
+ *   shape_f64[k*16 + 0..8]  R (collider2origin[:3,:3], row major)
+ *   shape_f64[k*16 + 9..11] t (collider2origin[:3,3])
+ *   shape_f64[k*16 + 12..14] p0,p1,p2 (type parameters)
+ *   shape_f64[k*16 + 15]    margin (Margin collider, 0 otherwise)
+ *   shape_i32[k*4 + 0..3]   type, n_vertices, vertex offset (FP32 pool),
+ *                           vertex offset in the FP64 pool or -1
+ *   vx, vy, vz              FP32 SoA vertex pool (local frame)
+ *   v64                     optional FP64 AoS pool (local frame, exact values)
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define D3D_HULL 0
+#define D3D_SPHERE 1
+#define D3D_CAPSULE 2
+#define D3D_ELLIPSOID 3
+#define D3D_CYLINDER 4
+#define D3D_BOX 5
+#define D3D_CONE 6
+#define D3D_DISK 7
+#define D3D_ELLIPSE 8
+
+#define ST_SEPARATED 0
+#define ST_INTERSECT 1
+#define ST_CLIPPED 3
+#define ST_ITER_CAP 4
+#define ST_SANITY 0x100
+
+#define EPS 2.220446049250313e-16 /* utils.py:8 */
+#define EPS_SQR (EPS * EPS)       /* _gjk_jolt.py:18 */
+#define MAXF DBL_MAX              /* utils.py:7 */
+
+typedef struct { double x, y, z; } V;
+
+typedef struct {
+    const double *f;
+    const int32_t *i;
+    const float *vx, *vy, *vz;
+    const double *v64;
+} Scene;
+
+static inline V mk(double x, double y, double z) { V r = {x, y, z}; return r; }
+static inline V add(V a, V b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V sub(V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V scl(V a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline V neg(V a) { return mk(-a.x, -a.y, -a.z); }
+static inline double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V cross(V a, V b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline double vget(V a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+/* ---------------------------------------------------------------- poses -- */
+
+/* local = R^T d, i.e. np.dot(A2B[:3, :3].T, d) (geometry.py:202) */
+static inline V to_local(const double *R, V d) {
+    return mk(R[0] * d.x + R[3] * d.y + R[6] * d.z,
+              R[1] * d.x + R[4] * d.y + R[7] * d.z,
+              R[2] * d.x + R[5] * d.y + R[8] * d.z);
+}
+
+/* transform_point: t + R p (utils.py:125-143) */
+static inline V to_world(const double *R, const double *t, V p) {
+    return mk(t[0] + (R[0] * p.x + R[1] * p.y + R[2] * p.z),
+              t[1] + (R[3] * p.x + R[4] * p.y + R[5] * p.z),
+              t[2] + (R[6] * p.x + R[7] * p.y + R[8] * p.z));
+}
+
+/* norm_vector (utils.py:12-30): v / |v|, or v itself when |v| == 0 */
+static inline V norm_vector(V v) {
+    double n = sqrt(dot(v, v));
+    if (n == 0.0) return v;
+    return mk(v.x / n, v.y / n, v.z / n);
+}
+
+static inline V hull_vertex(const Scene *s, int k, int64_t idx) {
+    const int32_t *I = s->i + 4 * (int64_t)k;
+    if (I[3] >= 0) {
+        const double *p = s->v64 + 3 * ((int64_t)I[3] + idx);
+        return mk(p[0], p[1], p[2]);
+    }
+    int64_t j = (int64_t)I[2] + idx;
+    return mk((double)s->vx[j], (double)s->vy[j], (double)s->vz[j]);
+}
+
+/* ------------------------------------------------------------- supports -- */
+
+/* ConvexHullVertices.support_function (colliders.py:131-132):
+ * vertices[argmax(vertices.dot(d))], first index on ties.  The argmax runs in
+ * the collider's local frame on R^T d; for a world-frame hull (identity pose)
+ * R^T d == d exactly and t + R v == v exactly, which is the reference's own
+ * expression. */
+static V support_hull(const Scene *s, int k, V d) {
+    const double *f = s->f + 16 * (int64_t)k;
+    int n = s->i[4 * (int64_t)k + 1];
+    V l = to_local(f, d);
+    double best = -INFINITY;
+    int bi = 0;
+    for (int j = 0; j < n; ++j) {
+        V v = hull_vertex(s, k, j);
+        double val = v.x * l.x + v.y * l.y + v.z * l.z;
+        if (val > best) { best = val; bi = j; }
+    }
+    return to_world(f, f + 9, hull_vertex(s, k, bi));
+}
+
+/* Box: argmax over convert_box_to_vertices (geometry.py:135-157,
+ * colliders.py:161-176).  BOX_COORDS enumerates product([-.5,.5], repeat=3),
+ * so among tied vertices the first has -0.5 on every tied axis. */
+static V support_box(const double *f, V d) {
+    V l = to_local(f, d);
+    V p = mk((l.x > 0.0 ? 0.5 : -0.5) * f[12], (l.y > 0.0 ? 0.5 : -0.5) * f[13],
+             (l.z > 0.0 ? 0.5 : -0.5) * f[14]);
+    return to_world(f, f + 9, p);
+}
+
+/* support_function_sphere (geometry.py:314-346) */
+static V support_sphere(const double *f, V d) {
+    V c = mk(f[9], f[10], f[11]);
+    double r = f[12];
+    double sn = sqrt(dot(d, d));
+    if (sn == 0.0) return add(c, mk(0.0, 0.0, r));
+    return add(c, mk(d.x / sn * r, d.y / sn * r, d.z / sn * r));
+}
+
+/* support_function_capsule (geometry.py:209-256) */
+static V support_capsule(const double *f, V d) {
+    V l = to_local(f, d);
+    double r = f[12], h = f[13];
+    double s = sqrt(l.x * l.x + l.y * l.y + l.z * l.z);
+    V p;
+    if (s == 0.0) p = mk(r, 0.0, 0.0);
+    else { double k = r / s; p = mk(l.x * k, l.y * k, l.z * k); }
+    if (l.z > 0.0) p.z += 0.5 * h;
+    else p.z -= 0.5 * h;
+    return to_world(f, f + 9, p);
+}
+
+/* support_function_ellipsoid (geometry.py:259-284) */
+static V support_ellipsoid(const double *f, V d) {
+    V l = to_local(f, d);
+    V r = mk(f[12], f[13], f[14]);
+    V q = norm_vector(mk(l.x * r.x, l.y * r.y, l.z * r.z));
+    return to_world(f, f + 9, mk(q.x * r.x, q.y * r.y, q.z * r.z));
+}
+
+/* support_function_cylinder (geometry.py:160-206) */
+static V support_cylinder(const double *f, V d) {
+    V l = to_local(f, d);
+    double r = f[12], L = f[13];
+    double s = sqrt(l.x * l.x + l.y * l.y);
+    double z = (l.z < 0.0) ? -0.5 * L : 0.5 * L;
+    V p;
+    if (s == 0.0) p = mk(r, 0.0, z);
+    else { double k = r / s; p = mk(l.x * k, l.y * k, z); }
+    return to_world(f, f + 9, p);
+}
+
+/* support_function_cone (geometry.py:417-454) */
+static V support_cone(const double *f, V d) {
+    V l = to_local(f, d);
+    double r = f[12], h = f[13];
+    V dp = mk(l.x, l.y, 0.0);
+    double n = sqrt(dot(dp, dp));
+    if (n == 0.0) dp = mk(0.0, 0.0, 0.0);
+    else { double k = r / n; dp = mk(dp.x * k, dp.y * k, dp.z * k); }
+    V p = (dot(l, dp) >= l.z * h) ? dp : mk(0.0, 0.0, h);
+    return to_world(f, f + 9, p);
+}
+
+/* support_function_disk (geometry.py:349-383); R = [x y n] holds
+ * plane_basis_from_normal(n) (utils.py:76-122) in its columns. */
+static V support_disk(const double *f, V d) {
+    V l = to_local(f, d);
+    l.z = 0.0;
+    double n = sqrt(dot(l, l));
+    if (n == 0.0) return mk(f[9], f[10], f[11]);
+    double k = f[12] / n;
+    l = mk(l.x * k, l.y * k, l.z * k);
+    return mk(f[9] + (f[0] * l.x + f[1] * l.y + f[2] * l.z),
+              f[10] + (f[3] * l.x + f[4] * l.y + f[5] * l.z),
+              f[11] + (f[6] * l.x + f[7] * l.y + f[8] * l.z));
+}
+
+/* support_function_ellipse (geometry.py:386-414); R's first two columns are
+ * the ellipse axes. */
+static V support_ellipse(const double *f, V d) {
+    double l0 = f[0] * d.x + f[3] * d.y + f[6] * d.z;
+    double l1 = f[1] * d.x + f[4] * d.y + f[7] * d.z;
+    double q0 = f[12] * l0, q1 = f[13] * l1;
+    double n = sqrt(q0 * q0 + q1 * q1);
+    if (n != 0.0) { q0 = q0 / n; q1 = q1 / n; }
+    double a0 = q0 * f[12], a1 = q1 * f[13];
+    return mk(f[9] + (a0 * f[0] + a1 * f[1]), f[10] + (a0 * f[3] + a1 * f[4]),
+              f[11] + (a0 * f[6] + a1 * f[7]));
+}
+
+/* collider.support_function(d); Margin adds margin * norm_vector(d)
+ * (colliders.py:629-631). */
+static V support(const Scene *s, int k, V d) {
+    const double *f = s->f + 16 * (int64_t)k;
+    V p;
+    switch (s->i[4 * (int64_t)k]) {
+    case D3D_HULL: p = support_hull(s, k, d); break;
+    case D3D_SPHERE: p = support_sphere(f, d); break;
+    case D3D_CAPSULE: p = support_capsule(f, d); break;
+    case D3D_ELLIPSOID: p = support_ellipsoid(f, d); break;
+    case D3D_CYLINDER: p = support_cylinder(f, d); break;
+    case D3D_BOX: p = support_box(f, d); break;
+    case D3D_CONE: p = support_cone(f, d); break;
+    case D3D_DISK: p = support_disk(f, d); break;
+    case D3D_ELLIPSE: p = support_ellipse(f, d); break;
+    default: p = mk(NAN, NAN, NAN); break;
+    }
+    if (f[15] != 0.0) p = add(p, scl(norm_vector(d), f[15]));
+    return p;
+}
+
+/* ------------------------------------------------- simplex sub-routines -- */
+
+/* get_barycentric_coordinates_line (_gjk_jolt.py:291-312) */
+static void bary_line(V a, V b, double *u, double *v) {
+    V ab = sub(b, a);
+    double den = dot(ab, ab);
+    if (den < EPS_SQR) {
+        if (dot(a, a) < dot(b, b)) { *u = 1.0; *v = 0.0; }
+        else { *u = 0.0; *v = 1.0; }
+    } else {
+        *v = -dot(a, ab) / den;
+        *u = 1.0 - *v;
+    }
+}
+
+/* get_barycentric_coordinates_plane (_gjk_jolt.py:315-372) */
+static void bary_plane(V a, V b, V c, double *u, double *v, double *w) {
+    V v0 = sub(b, a), v1 = sub(c, a), v2 = sub(c, b);
+    double d00 = dot(v0, v0), d11 = dot(v1, v1), d22 = dot(v2, v2);
+    if (d00 <= d22) {
+        double d01 = dot(v0, v1);
+        double den = d00 * d11 - d01 * d01;
+        if (fabs(den) < EPS) {
+            if (d00 > d11) { bary_line(a, b, u, v); *w = 0.0; }
+            else { bary_line(a, c, u, w); *v = 0.0; }
+        } else {
+            double a0 = dot(a, v0), a1 = dot(a, v1);
+            *v = (d01 * a1 - d11 * a0) / den;
+            *w = (d01 * a0 - d00 * a1) / den;
+            *u = 1.0 - *v - *w;
+        }
+    } else {
+        double d12 = dot(v1, v2);
+        double den = d11 * d22 - d12 * d12;
+        if (fabs(den) < EPS) {
+            if (d11 > d22) { bary_line(a, c, u, w); *v = 0.0; }
+            else { bary_line(b, c, v, w); *u = 0.0; }
+        } else {
+            double c1 = dot(c, v1), c2 = dot(c, v2);
+            *u = (d22 * c1 - d12 * c2) / den;
+            *v = (d11 * c2 - d12 * c1) / den;
+            *w = 1.0 - *u - *v;
+        }
+    }
+}
+
+/* scalar_triple_product a . (b x c) (utils.py:33-73) */
+static inline double stp(V a, V b, V c) { return dot(a, cross(b, c)); }
+
+/* get_barycentric_coordinates_tetrahedron (_gjk_jolt.py:375-390) */
+static void bary_tetra(V a, V b, V c, V d, double *u, double *v, double *w, double *x) {
+    V vab = sub(b, a), vac = sub(c, a), vad = sub(d, a);
+    double va6 = -stp(b, sub(d, b), sub(c, b));
+    double vb6 = -stp(a, vac, vad);
+    double vc6 = -stp(a, vad, vab);
+    double vd6 = -stp(a, vab, vac);
+    double v6 = 1.0 / stp(vab, vac, vad);
+    *u = va6 * v6; *v = vb6 * v6; *w = vc6 * v6; *x = vd6 * v6;
+}
+
+/* closest_point_line (_gjk_jolt.py:393-412) */
+static V cp_line(V a, V b, int *set) {
+    double u, v;
+    bary_line(a, b, &u, &v);
+    if (v <= 0.0) { *set = 1; return a; }
+    if (u <= 0.0) { *set = 2; return b; }
+    *set = 3;
+    return add(scl(a, u), scl(b, v));
+}
+
+/* closest_point_triangle (_gjk_jolt.py:415-523) */
+static V cp_triangle(V a, V b, V c, int *set) {
+    V ab = sub(b, a), ac = sub(c, a), bc = sub(c, b);
+    V n = (dot(bc, bc) < dot(ac, ac)) ? cross(ab, bc) : cross(ab, ac);
+    double nn = dot(n, n);
+    if (nn < EPS_SQR) {
+        int s, ns;
+        V cp = cp_line(a, b, &s);
+        double best = dot(cp, cp);
+        V q = cp_line(a, c, &ns);
+        double dd = dot(q, q);
+        if (dd < best) { cp = q; best = dd; s = (ns & 1) + ((ns & 2) << 1); }
+        q = cp_line(b, c, &ns);
+        dd = dot(q, q);
+        if (dd < best) { cp = q; s = ns << 1; }
+        *set = s;
+        return cp;
+    }
+    V ap = neg(a);
+    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) { *set = 1; return a; }
+    V bp = neg(b);
+    double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) { *set = 2; return b; }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && 0.0 <= d1 && d3 <= 0.0) {
+        double v = d1 / (d1 - d3);
+        *set = 3;
+        return add(a, scl(ab, v));
+    }
+    V cpt = neg(c);
+    double d5 = dot(ab, cpt), d6 = dot(ac, cpt);
+    if (d6 >= 0.0 && d5 <= d6) { *set = 4; return c; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && 0.0 <= d2 && d6 <= 0.0) {
+        double w = d2 / (d2 - d6);
+        *set = 5;
+        return add(a, scl(ac, w));
+    }
+    double va = d3 * d6 - d5 * d4;
+    double d4_d3 = d4 - d3, d5_d6 = d5 - d6;
+    if (va <= 0.0 && 0.0 <= d4_d3 && d5_d6 >= 0.0) {
+        double w = d4_d3 / (d4_d3 + d5_d6);
+        *set = 6;
+        return add(b, scl(bc, w));
+    }
+    /* n * (a + b + c).dot(n) / (3.0 * n_len_sq) */
+    double s = dot(add(add(a, b), c), n);
+    double den = 3.0 * nn;
+    *set = 7;
+    return mk(n.x * s / den, n.y * s / den, n.z * s / den);
+}
+
+/* origin_outside_of_tetrahedron_planes (_gjk_jolt.py:526-570) */
+static void outside_planes(V a, V b, V c, V d, int out[4]) {
+    V ab = sub(b, a), ac = sub(c, a), ad = sub(d, a), bd = sub(d, b), bc = sub(c, b);
+    V abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab), bdc = cross(bd, bc);
+    double sp[4] = {dot(a, abc), dot(a, acd), dot(a, adb), dot(b, bdc)};
+    double sd[4] = {dot(ad, abc), dot(ab, acd), dot(ac, adb), -dot(ab, bdc)};
+    int allpos = sd[0] > 0.0 && sd[1] > 0.0 && sd[2] > 0.0 && sd[3] > 0.0;
+    int allneg = sd[0] < 0.0 && sd[1] < 0.0 && sd[2] < 0.0 && sd[3] < 0.0;
+    for (int k = 0; k < 4; ++k) {
+        if (allpos) out[k] = sp[k] >= -EPS;
+        else if (allneg) out[k] = sp[k] <= EPS;
+        else out[k] = 1;
+    }
+}
+
+/* closest_point_tetrahedron (_gjk_jolt.py:573-631) */
+static V cp_tetra(V a, V b, V c, V d, int *set) {
+    int s = 15;
+    V cp = mk(0.0, 0.0, 0.0);
+    double best = MAXF;
+    int o[4];
+    outside_planes(a, b, c, d, o);
+    int ns;
+    if (o[0]) { cp = cp_triangle(a, b, c, &s); best = dot(cp, cp); }
+    if (o[1]) {
+        V q = cp_triangle(a, c, d, &ns);
+        double dd = dot(q, q);
+        if (dd < best) { best = dd; cp = q; s = (ns & 1) + ((ns & 6) << 1); }
+    }
+    if (o[2]) {
+        V q = cp_triangle(a, d, b, &ns);
+        double dd = dot(q, q);
+        if (dd < best) { best = dd; cp = q; s = (ns & 1) + ((ns & 2) << 2) + ((ns & 4) >> 1); }
+    }
+    if (o[3]) {
+        V q = cp_triangle(b, d, c, &ns);
+        double dd = dot(q, q);
+        if (dd < best) { cp = q; s = ((ns & 1) << 1) + ((ns & 2) << 2) + (ns & 4); }
+    }
+    *set = s;
+    return cp;
+}
+
+/* get_closest_point_to_origin (_gjk_jolt.py:690-711) */
+static int closest_to_origin(const V *Y, int n, double prev, V *v, double *vv, int *set) {
+    V p;
+    int s;
+    if (n == 1) { s = 1; p = Y[0]; }
+    else if (n == 2) p = cp_line(Y[0], Y[1], &s);
+    else if (n == 3) p = cp_triangle(Y[0], Y[1], Y[2], &s);
+    else p = cp_tetra(Y[0], Y[1], Y[2], Y[3], &s);
+    double l = dot(p, p);
+    if (l < prev) { *v = p; *vv = l; *set = s; return 1; }
+    return 0;
+}
+
+/* max_y_length_squared (_gjk_jolt.py:634-640) */
+static double max_y_len_sq(const V *Y, int n) {
+    double m = dot(Y[0], Y[0]);
+    for (int k = 1; k < n; ++k) { double l = dot(Y[k], Y[k]); m = (l > m) ? l : m; }
+    return m;
+}
+
+/* update_simplex_ypq (_gjk_jolt.py:654-664) */
+static int compact(V *Y, V *P, V *Q, int n, int set) {
+    int m = 0;
+    for (int k = 0; k < n; ++k)
+        if (set & (1 << k)) { Y[m] = Y[k]; if (P) { P[m] = P[k]; Q[m] = Q[k]; } ++m; }
+    return m;
+}
+
+/* calculate_closest_points (_gjk_jolt.py:667-687) */
+static void closest_points(const V *Y, const V *P, const V *Q, int n, V *a, V *b) {
+    if (n == 1) { *a = P[0]; *b = Q[0]; }
+    else if (n == 2) {
+        double u, v;
+        bary_line(Y[0], Y[1], &u, &v);
+        *a = add(scl(P[0], u), scl(P[1], v));
+        *b = add(scl(Q[0], u), scl(Q[1], v));
+    } else if (n == 3) {
+        double u, v, w;
+        bary_plane(Y[0], Y[1], Y[2], &u, &v, &w);
+        *a = add(add(scl(P[0], u), scl(P[1], v)), scl(P[2], w));
+        *b = add(add(scl(Q[0], u), scl(Q[1], v)), scl(Q[2], w));
+    } else {
+        double u, v, w, x;
+        bary_tetra(Y[0], Y[1], Y[2], Y[3], &u, &v, &w, &x);
+        *a = add(add(add(scl(P[0], u), scl(P[1], v)), scl(P[2], w)), scl(P[3], x));
+        *b = add(add(add(scl(Q[0], u), scl(Q[1], v)), scl(Q[2], w)), scl(Q[3], x));
+    }
+}
+
+/* ---------------------------------------------------------------- GJK -- */
+
+typedef struct {
+    double dist;
+    V a, b;
+    V Y[4], P[4], Q[4];
+    int n_points;
+    int status;
+    int iterations;
+} GjkOut;
+
+/* gjk_distance_jolt (_gjk_jolt.py:138-221) with _distance_loop (:224-288).
+ * The reference loop is unbounded; here it stops after max_iter iterations
+ * with status ST_ITER_CAP. */
+static void gjk_distance_pair(const Scene *s, int ka, int kb, double tol, double maxd2,
+                              double sanity, int max_iter, GjkOut *o) {
+    V *Y = o->Y, *P = o->P, *Q = o->Q;
+    memset(Y, 0, sizeof(o->Y)); memset(P, 0, sizeof(o->P)); memset(Q, 0, sizeof(o->Q));
+    int n = 0;
+    double tol2 = tol * tol;
+    V d = mk(1.0, 0.0, 0.0);
+    double vv = dot(d, d);
+    double prev = MAXF;
+    int state = -1, it = 0;
+    while (state < 0) {
+        if (it >= max_iter) { state = ST_ITER_CAP; break; }
+        ++it;
+        V p = support(s, ka, d);
+        V q = support(s, kb, neg(d));
+        V w = sub(p, q);
+        double dw = dot(d, w);
+        if (dw < 0.0 && dw * dw > vv * maxd2) { state = ST_CLIPPED; break; }
+        Y[n] = w; P[n] = p; Q[n] = q; ++n;
+        V nv; double nvv; int set;
+        if (closest_to_origin(Y, n, prev, &nv, &nvv, &set)) { d = nv; vv = nvv; }
+        else { --n; set = 0; for (int k = 0; k < n; ++k) set |= 1 << k; }
+        if (set == 15) { vv = 0.0; state = ST_INTERSECT; break; }
+        n = compact(Y, P, Q, n, set);
+        if (vv <= tol2) { vv = 0.0; state = ST_INTERSECT; break; }
+        if (vv <= EPS * max_y_len_sq(Y, n)) { vv = 0.0; state = ST_INTERSECT; break; }
+        d = neg(d);
+        if (prev - vv <= EPS * prev) { state = ST_SEPARATED; break; }
+        prev = vv;
+    }
+    o->iterations = it;
+    o->n_points = n;
+    if (state == ST_CLIPPED) {
+        o->status = ST_CLIPPED;
+        o->dist = MAXF;
+        o->a = o->b = mk(NAN, NAN, NAN);
+        return;
+    }
+    V a, b;
+    closest_points(Y, P, Q, n, &a, &b);
+    o->status = state;
+    double check = fabs(dot(d, d) - vv);
+    if (!(check < sanity)) o->status |= ST_SANITY;
+    double dist = sqrt(vv);
+    if (dist < EPS) { V m = scl(add(a, b), 0.5); a = m; b = m; }
+    o->dist = dist;
+    o->a = a;
+    o->b = b;
+}
+
+/* gjk_intersection_jolt (_gjk_jolt.py:29-80) with _intersection_loop
+ * (:83-135).  Returns 1 (intersection), 0 (separated), or -1 on the
+ * iteration cap. */
+static int gjk_intersection_pair(const Scene *s, int ka, int kb, double tol, int max_iter,
+                                 int *iters) {
+    V Y[4];
+    int n = 0;
+    double tol2 = tol * tol, prev = MAXF;
+    V d = mk(1.0, 0.0, 0.0);
+    for (int it = 0; it < max_iter; ++it) {
+        if (iters) *iters = it + 1;
+        V w = sub(support(s, ka, d), support(s, kb, neg(d)));
+        if (dot(d, w) < -EPS) return 0;
+        Y[n++] = w;
+        V v; double vv; int set;
+        if (!closest_to_origin(Y, n, prev, &v, &vv, &set)) return 0;
+        if (set == 15) return 1;
+        if (vv <= tol2) return 1;
+        if (vv <= EPS * max_y_len_sq(Y, n)) return 1;
+        d = neg(v);
+        if (prev - vv <= EPS * prev) return 0;
+        prev = vv;
+        n = compact(Y, NULL, NULL, n, set);
+    }
+    return -1;
+}
+
+/* ---------------------------------------------------------------- EPA -- */
+/*
+ * Expanding polytope (epa.py:9-202), restated with the changes that make it
+ * well defined for every GJK exit:
+ *   - the seed tetrahedron is oriented (each face normal points away from the
+ *     opposite vertex) instead of taking the simplex rows unoriented
+ *     (epa.py:89-97);
+ *   - GJK simplices with fewer than 4 points are blown up to a tetrahedron
+ *     that still contains the origin (epa.py always reads 4 rows);
+ *   - faces reference polytope vertices by index, so horizon edges are
+ *     matched exactly instead of by position within epsilon (epa.py:189-191),
+ *     and new faces inherit the removed faces' winding (no fix_ccw flip);
+ *   - the support points on both colliders are kept per vertex, so the
+ *     result carries contact points on A and B;
+ *   - exceeding max_faces ends the loop with success = 0 instead of raising
+ *     (epa.py:128).
+ * Faces are kept in a stable order: surviving faces keep their order, new
+ * faces follow in horizon order (removed-face order, then edge 0, 1, 2).
+ * The closest face is the first one with the smallest f0 . n
+ * (epa.py:104-109); convergence is w . n - dist < epsilon (epa.py:67-70).
+ */
+#define EPA_MAX_VERTS 260
+#define EPA_MAX_FACES_CAP 512
+
+typedef struct {
+    V mtv, normal, ca, cb;
+    double depth;
+    int status; /* 1 converged, 0 not converged, 2 face cap, 3 degenerate seed */
+    int iterations;
+    int n_faces;
+} EpaOut;
+
+typedef struct { int v[3]; V n; double dist; } Face;
+
+static int epa_add_vertex(const Scene *s, int ka, int kb, V d, V *W, V *PA, V *PB, int *nv) {
+    V pa = support(s, ka, d), pb = support(s, kb, neg(d));
+    W[*nv] = sub(pa, pb); PA[*nv] = pa; PB[*nv] = pb;
+    return (*nv)++;
+}
+
+static int make_face(Face *f, const V *W, int i0, int i1, int i2) {
+    f->v[0] = i0; f->v[1] = i1; f->v[2] = i2;
+    V n = cross(sub(W[i1], W[i0]), sub(W[i2], W[i0]));
+    double l = sqrt(dot(n, n));
+    if (!(l > 0.0)) return 0;
+    f->n = mk(n.x / l, n.y / l, n.z / l);
+    f->dist = dot(W[i0], f->n);
+    return 1;
+}
+
+/* Seed tetrahedron containing the origin (on or inside) from a GJK simplex of
+ * n points.  Returns 1 on success with the 4 vertex indices in W[0..3]. */
+static int epa_seed(const Scene *s, int ka, int kb, const V *Y, const V *P, const V *Q, int n,
+                    V *W, V *PA, V *PB, int *nv) {
+    *nv = 0;
+    if (n < 1 || n > 4) return 0;
+    for (int k = 0; k < n; ++k) { W[k] = Y[k]; PA[k] = P[k]; PB[k] = Q[k]; }
+    *nv = n;
+    if (n == 1) {
+        /* origin at (or within tolerance of) W[0]: extend along the axis on
+         * which the support differs most from W[0] */
+        int best = -1; double bl = 0.0; V bw = W[0], bpa = PA[0], bpb = PB[0];
+        static const double dirs[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+        for (int k = 0; k < 6; ++k) {
+            V d = mk(dirs[k][0], dirs[k][1], dirs[k][2]);
+            V pa = support(s, ka, d), pb = support(s, kb, neg(d));
+            V w = sub(pa, pb);
+            V e = sub(w, W[0]);
+            double l = dot(e, e);
+            if (l > bl) { bl = l; best = k; bw = w; bpa = pa; bpb = pb; }
+        }
+        if (best < 0 || !(bl > EPS)) return 0;
+        W[1] = bw; PA[1] = bpa; PB[1] = bpb; n = 2; *nv = 2;
+    }
+    if (n == 2) {
+        V ab = sub(W[1], W[0]);
+        /* perpendicular to ab: cross with the axis of its smallest component */
+        double ax = fabs(ab.x), ay = fabs(ab.y), az = fabs(ab.z);
+        V e = (ax <= ay && ax <= az) ? mk(1, 0, 0) : ((ay <= az) ? mk(0, 1, 0) : mk(0, 0, 1));
+        V p1 = cross(ab, e);
+        V p2 = cross(ab, p1);
+        V cand[4] = {p1, neg(p1), p2, neg(p2)};
+        int best = -1; double bl = 0.0; V bw = W[0], bpa = PA[0], bpb = PB[0];
+        for (int k = 0; k < 4; ++k) {
+            V pa = support(s, ka, cand[k]), pb = support(s, kb, neg(cand[k]));
+            V w = sub(pa, pb);
+            V c = cross(ab, sub(w, W[0]));
+            double l = dot(c, c);
+            if (l > bl) { bl = l; best = k; bw = w; bpa = pa; bpb = pb; }
+        }
+        if (best < 0 || !(bl > EPS * dot(ab, ab))) return 0;
+        W[2] = bw; PA[2] = bpa; PB[2] = bpb; n = 3; *nv = 3;
+    }
+    if (n == 3) {
+        V nrm = cross(sub(W[1], W[0]), sub(W[2], W[0]));
+        double nl = dot(nrm, nrm);
+        if (!(nl > 0.0)) return 0;
+        V wp_pa = support(s, ka, nrm), wp_pb = support(s, kb, neg(nrm));
+        V wp = sub(wp_pa, wp_pb);
+        V wm_pa = support(s, ka, neg(nrm)), wm_pb = support(s, kb, nrm);
+        V wm = sub(wm_pa, wm_pb);
+        double hp = dot(sub(wp, W[0]), nrm), hm = -dot(sub(wm, W[0]), nrm);
+        if (hp >= hm) { W[3] = wp; PA[3] = wp_pa; PB[3] = wp_pb; }
+        else { W[3] = wm; PA[3] = wm_pa; PB[3] = wm_pb; }
+        if (!((hp > hm ? hp : hm) > 0.0)) return 0;
+        *nv = 4;
+    }
+    return 1;
+}
+
+static void epa_pair(const Scene *s, int ka, int kb, const V *Y, const V *P, const V *Q, int n,
+                     int max_iter, int max_faces, double eps, EpaOut *o) {
+    V W[EPA_MAX_VERTS], PA[EPA_MAX_VERTS], PB[EPA_MAX_VERTS];
+    Face F[EPA_MAX_FACES_CAP], NF[EPA_MAX_FACES_CAP];
+    int nv = 0;
+    if (max_faces > EPA_MAX_FACES_CAP) max_faces = EPA_MAX_FACES_CAP;
+    if (max_iter > EPA_MAX_VERTS - 8) max_iter = EPA_MAX_VERTS - 8;
+    memset(o, 0, sizeof(*o));
+    if (!epa_seed(s, ka, kb, Y, P, Q, n, W, PA, PB, &nv)) { o->status = 3; return; }
+    /* orient the seed: vertex 3 must be behind face (0,1,2) */
+    {
+        V nrm = cross(sub(W[1], W[0]), sub(W[2], W[0]));
+        if (dot(sub(W[3], W[0]), nrm) > 0.0) {
+            V t = W[1]; W[1] = W[2]; W[2] = t;
+            t = PA[1]; PA[1] = PA[2]; PA[2] = t;
+            t = PB[1]; PB[1] = PB[2]; PB[2] = t;
+        }
+    }
+    int nf = 0;
+    static const int seed_faces[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+    for (int k = 0; k < 4; ++k)
+        if (make_face(&F[nf], W, seed_faces[k][0], seed_faces[k][1], seed_faces[k][2])) ++nf;
+    if (nf < 4) { o->status = 3; return; }
+
+    int status = 0, it = 0, best = 0;
+    V w = mk(0, 0, 0);
+    for (it = 0; it < max_iter; ++it) {
+        best = 0;
+        for (int k = 1; k < nf; ++k) if (F[k].dist < F[best].dist) best = k;
+        Face cf = F[best];
+        V pa = support(s, ka, cf.n), pb = support(s, kb, neg(cf.n));
+        w = sub(pa, pb);
+        double wn = dot(w, cf.n);
+        if (wn - cf.dist < eps) {
+            status = 1;
+            o->depth = wn;
+            o->normal = cf.n;
+            o->mtv = scl(cf.n, wn);
+            break;
+        }
+        if (nv >= EPA_MAX_VERTS) { status = 2; break; }
+        int wi = nv;
+        W[nv] = w; PA[nv] = pa; PB[nv] = pb; ++nv;
+        /* faces that see w */
+        int removed[EPA_MAX_FACES_CAP];
+        int nrem = 0;
+        for (int k = 0; k < nf; ++k) {
+            removed[k] = dot(F[k].n, sub(w, W[F[k].v[0]])) > eps;
+            nrem += removed[k];
+        }
+        /* horizon edges: edges of removed faces whose reverse is not an edge
+         * of another removed face */
+        int he[3 * EPA_MAX_FACES_CAP][2];
+        int nh = 0;
+        for (int k = 0; k < nf; ++k) {
+            if (!removed[k]) continue;
+            for (int e = 0; e < 3; ++e) {
+                int a = F[k].v[e], b = F[k].v[(e + 1) % 3];
+                int shared = 0;
+                for (int m = 0; m < nf && !shared; ++m) {
+                    if (m == k || !removed[m]) continue;
+                    for (int g = 0; g < 3; ++g)
+                        if (F[m].v[g] == b && F[m].v[(g + 1) % 3] == a) { shared = 1; break; }
+                }
+                if (!shared) { he[nh][0] = a; he[nh][1] = b; ++nh; }
+            }
+        }
+        int nn = 0;
+        for (int k = 0; k < nf; ++k) if (!removed[k]) NF[nn++] = F[k];
+        int cap = 0;
+        for (int h = 0; h < nh; ++h) {
+            if (nn >= max_faces) { cap = 1; break; }
+            if (make_face(&NF[nn], W, he[h][0], he[h][1], wi)) ++nn;
+        }
+        memcpy(F, NF, sizeof(Face) * nn);
+        nf = nn;
+        if (cap || nf == 0) { status = 2; break; }
+    }
+    if (status != 1) {
+        /* closest face of the current polytope: mtv = n * (f0 . n)
+         * (epa.py:76-78) */
+        best = 0;
+        for (int k = 1; k < nf; ++k) if (F[k].dist < F[best].dist) best = k;
+        if (nf > 0) {
+            o->depth = F[best].dist;
+            o->normal = F[best].n;
+            o->mtv = scl(F[best].n, F[best].dist);
+        }
+    }
+    o->status = status;
+    o->iterations = it + (status == 1 ? 1 : 0);
+    o->n_faces = nf;
+    if (nf > 0) {
+        /* contact points: barycentric coordinates of depth * n in the chosen
+         * face, applied to the support points on A and B */
+        Face cf = F[best];
+        V pt = scl(cf.n, o->depth);
+        V a = W[cf.v[0]], b = W[cf.v[1]], c = W[cf.v[2]];
+        V v0 = sub(b, a), v1 = sub(c, a), v2 = sub(pt, a);
+        double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
+        double d20 = dot(v2, v0), d21 = dot(v2, v1);
+        double den = d00 * d11 - d01 * d01;
+        double bv = 0.0, bw = 0.0;
+        if (den != 0.0) { bv = (d11 * d20 - d01 * d21) / den; bw = (d00 * d21 - d01 * d20) / den; }
+        double bu = 1.0 - bv - bw;
+        o->ca = add(add(scl(PA[cf.v[0]], bu), scl(PA[cf.v[1]], bv)), scl(PA[cf.v[2]], bw));
+        o->cb = add(add(scl(PB[cf.v[0]], bu), scl(PB[cf.v[1]], bv)), scl(PB[cf.v[2]], bw));
+    }
+}
+
+/* --------------------------------------------------------------- AABBs -- */
+
+/* collider.aabb() as (3,2) [axis][min,max], containment.py:6-229 */
+static void aabb_of(const Scene *s, int k, double out[6]) {
+    const double *f = s->f + 16 * (int64_t)k;
+    const int32_t *I = s->i + 4 * (int64_t)k;
+    const double *R = f, *t = f + 9;
+    double mn[3], mx[3];
+    switch (I[0]) {
+    case D3D_HULL: {
+        /* axis_aligned_bounding_box of the world vertices (containment.py:6-22) */
+        for (int a = 0; a < 3; ++a) { mn[a] = INFINITY; mx[a] = -INFINITY; }
+        for (int j = 0; j < I[1]; ++j) {
+            V w = to_world(R, t, hull_vertex(s, k, j));
+            for (int a = 0; a < 3; ++a) {
+                double c = vget(w, a);
+                if (c < mn[a]) mn[a] = c;
+                if (c > mx[a]) mx[a] = c;
+            }
+        }
+        break;
+    }
+    case D3D_BOX: {
+        /* box_aabb over convert_box_to_vertices (containment.py:47-67,
+         * geometry.py:138-157): t + (BOX_COORDS * size) . R^T */
+        for (int a = 0; a < 3; ++a) { mn[a] = INFINITY; mx[a] = -INFINITY; }
+        for (int c = 0; c < 8; ++c) {
+            double l0 = ((c >> 2) & 1 ? 0.5 : -0.5) * f[12];
+            double l1 = ((c >> 1) & 1 ? 0.5 : -0.5) * f[13];
+            double l2 = (c & 1 ? 0.5 : -0.5) * f[14];
+            for (int a = 0; a < 3; ++a) {
+                double v = t[a] + (l0 * R[3 * a] + l1 * R[3 * a + 1] + l2 * R[3 * a + 2]);
+                if (v < mn[a]) mn[a] = v;
+                if (v > mx[a]) mx[a] = v;
+            }
+        }
+        break;
+    }
+    case D3D_SPHERE:
+        for (int a = 0; a < 3; ++a) { mn[a] = t[a] - f[12]; mx[a] = t[a] + f[12]; }
+        break;
+    case D3D_CAPSULE:
+        /* 0.5 * height * |R[:, 2]| + radius (containment.py:99-122) */
+        for (int a = 0; a < 3; ++a) {
+            double e = 0.5 * f[13] * fabs(R[3 * a + 2]) + f[12];
+            mn[a] = t[a] - e; mx[a] = t[a] + e;
+        }
+        break;
+    case D3D_CYLINDER:
+        /* 0.5 * length * |axis| + radius * sqrt(1 - axis^2) (containment.py:70-96) */
+        for (int a = 0; a < 3; ++a) {
+            double ax = R[3 * a + 2];
+            double e = 0.5 * f[13] * fabs(ax) + f[12] * sqrt(1.0 - ax * ax);
+            mn[a] = t[a] - e; mx[a] = t[a] + e;
+        }
+        break;
+    case D3D_ELLIPSOID: {
+        /* containment.py:125-148 */
+        double E[3][3];
+        double r[3] = {f[12], f[13], f[14]};
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) E[i][j] = R[3 * i + j] * r[j];
+        for (int j = 0; j < 3; ++j) {
+            double cn = sqrt(E[0][j] * E[0][j] + E[1][j] * E[1][j] + E[2][j] * E[2][j]);
+            for (int i = 0; i < 3; ++i) E[i][j] = E[i][j] / cn;
+        }
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) E[i][j] = E[i][j] * r[j];
+        for (int kk = 0; kk < 3; ++kk) {
+            double m = -INFINITY;
+            for (int i = 0; i < 3; ++i) {
+                double v = R[3 * i] * E[kk][0] + R[3 * i + 1] * E[kk][1] + R[3 * i + 2] * E[kk][2];
+                if (v > m) m = v;
+            }
+            mn[kk] = t[kk] - m; mx[kk] = t[kk] + m;
+        }
+        break;
+    }
+    case D3D_CONE: {
+        /* containment.py:177-203 */
+        double h = f[13], r = f[12];
+        for (int a = 0; a < 3; ++a) {
+            double pa = t[a], pb = t[a] + h * R[3 * a + 2];
+            double ad = pb - pa;
+            double e = sqrt(1.0 - ad * ad / (h * h));
+            double lo = pa - e * r, hi = pa + e * r;
+            mn[a] = lo < pb ? lo : pb;
+            mx[a] = hi > pb ? hi : pb;
+        }
+        break;
+    }
+    case D3D_DISK:
+        /* radius * sqrt(1 - normal^2) (containment.py:151-174) */
+        for (int a = 0; a < 3; ++a) {
+            double nrm = R[3 * a + 2];
+            double e = f[12] * sqrt(1.0 - nrm * nrm);
+            mn[a] = t[a] - e; mx[a] = t[a] + e;
+        }
+        break;
+    case D3D_ELLIPSE:
+        /* sqrt((r0 * ax0)^2 + (r1 * ax1)^2) (containment.py:206-229) */
+        for (int a = 0; a < 3; ++a) {
+            double u = f[12] * R[3 * a], v = f[13] * R[3 * a + 1];
+            double e = sqrt(u * u + v * v);
+            mn[a] = t[a] - e; mx[a] = t[a] + e;
+        }
+        break;
+    default:
+        for (int a = 0; a < 3; ++a) { mn[a] = NAN; mx[a] = NAN; }
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (f[15] != 0.0) { mn[a] = mn[a] - f[15]; mx[a] = mx[a] + f[15]; }
+        out[2 * a] = mn[a];
+        out[2 * a + 1] = mx[a];
+    }
+}
+
+/* aabb_overlap (aabb_tree.py:503-527): closed intervals on all three axes */
+static inline int aabb_overlap(const double *a, const double *b) {
+    return a[0] <= b[1] && a[1] >= b[0] && a[2] <= b[3] && a[3] >= b[2] && a[4] <= b[5] &&
+           a[5] >= b[4];
+}
+
+/* ----------------------------------------------------------- C exports -- */
+
+static Scene mkscene(const double *f, const int32_t *i, const float *vx, const float *vy,
+                     const float *vz, const double *v64) {
+    Scene s = {f, i, vx, vy, vz, v64};
+    return s;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+void orc_support(const double *f, const int32_t *i, const float *vx, const float *vy,
+                 const float *vz, const double *v64, const int32_t *shape, const double *dirs,
+                 int64_t n, double *out) {
+    Scene s = mkscene(f, i, vx, vy, vz, v64);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < n; ++k) {
+        V p = support(&s, shape[k], mk(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2]));
+        out[3 * k] = p.x; out[3 * k + 1] = p.y; out[3 * k + 2] = p.z;
+    }
+}
+
+void orc_aabbs(const double *f, const int32_t *i, const float *vx, const float *vy,
+               const float *vz, const double *v64, const int32_t *shape, int64_t n,
+               double *out) {
+    Scene s = mkscene(f, i, vx, vy, vz, v64);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t k = 0; k < n; ++k) aabb_of(&s, shape[k], out + 6 * k);
+}
+
+/* Batched GJK distance.  simplex (optional) receives Y, P, Q (3 x 4 x 3) and
+ * n_points per pair. */
+void orc_gjk_distance(const double *f, const int32_t *i, const float *vx, const float *vy,
+                      const float *vz, const double *v64, const int32_t *pa, const int32_t *pb,
+                      int64_t n, double tol, double maxd2, double sanity, int max_iter,
+                      double *dist, double *cpa, double *cpb, int32_t *status, int32_t *iters,
+                      double *simplex, int32_t *n_points) {
+    Scene s = mkscene(f, i, vx, vy, vz, v64);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t k = 0; k < n; ++k) {
+        GjkOut o;
+        gjk_distance_pair(&s, pa[k], pb[k], tol, maxd2, sanity, max_iter, &o);
+        dist[k] = o.dist;
+        cpa[3 * k] = o.a.x; cpa[3 * k + 1] = o.a.y; cpa[3 * k + 2] = o.a.z;
+        cpb[3 * k] = o.b.x; cpb[3 * k + 1] = o.b.y; cpb[3 * k + 2] = o.b.z;
+        status[k] = o.status;
+        if (iters) iters[k] = o.iterations;
+        if (simplex) {
+            double *S = simplex + 36 * k;
+            for (int r = 0; r < 4; ++r) {
+                S[3 * r] = o.Y[r].x; S[3 * r + 1] = o.Y[r].y; S[3 * r + 2] = o.Y[r].z;
+                S[12 + 3 * r] = o.P[r].x; S[13 + 3 * r] = o.P[r].y; S[14 + 3 * r] = o.P[r].z;
+                S[24 + 3 * r] = o.Q[r].x; S[25 + 3 * r] = o.Q[r].y; S[26 + 3 * r] = o.Q[r].z;
+            }
+        }
+        if (n_points) n_points[k] = o.n_points;
+    }
+}
+
+void orc_gjk_intersection(const double *f, const int32_t *i, const float *vx, const float *vy,
+                          const float *vz, const double *v64, const int32_t *pa,
+                          const int32_t *pb, int64_t n, double tol, int max_iter,
+                          int32_t *hit, int32_t *iters) {
+    Scene s = mkscene(f, i, vx, vy, vz, v64);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t k = 0; k < n; ++k) {
+        int it = 0;
+        hit[k] = gjk_intersection_pair(&s, pa[k], pb[k], tol, max_iter, &it);
+        if (iters) iters[k] = it;
+    }
+}
+
+/* Batched EPA seeded from GJK simplices (layout of orc_gjk_distance).
+ * out per pair: depth, normal(3), mtv(3), ca(3), cb(3) = 13 doubles. */
+void orc_epa(const double *f, const int32_t *i, const float *vx, const float *vy,
+             const float *vz, const double *v64, const int32_t *pa, const int32_t *pb,
+             int64_t n, const double *simplex, const int32_t *n_points, int max_iter,
+             int max_faces, double eps, double *out, int32_t *status, int32_t *iters) {
+    Scene s = mkscene(f, i, vx, vy, vz, v64);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t k = 0; k < n; ++k) {
+        const double *S = simplex + 36 * k;
+        V Y[4], P[4], Q[4];
+        for (int r = 0; r < 4; ++r) {
+            Y[r] = mk(S[3 * r], S[3 * r + 1], S[3 * r + 2]);
+            P[r] = mk(S[12 + 3 * r], S[13 + 3 * r], S[14 + 3 * r]);
+            Q[r] = mk(S[24 + 3 * r], S[25 + 3 * r], S[26 + 3 * r]);
+        }
+        EpaOut o;
+        epa_pair(&s, pa[k], pb[k], Y, P, Q, n_points[k], max_iter, max_faces, eps, &o);
+        double *O = out + 13 * k;
+        O[0] = o.depth;
+        O[1] = o.normal.x; O[2] = o.normal.y; O[3] = o.normal.z;
+        O[4] = o.mtv.x; O[5] = o.mtv.y; O[6] = o.mtv.z;
+        O[7] = o.ca.x; O[8] = o.ca.y; O[9] = o.ca.z;
+        O[10] = o.cb.x; O[11] = o.cb.y; O[12] = o.cb.z;
+        status[k] = o.status;
+        if (iters) iters[k] = o.iterations;
+    }
+}
+
+/* Sub-routines exposed for the closed-form / brute-force pin tests. */
+void orc_closest_point_simplex(const double *pts, int n, double *out, int32_t *set) {
+    V Y[4];
+    for (int k = 0; k < n; ++k) Y[k] = mk(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+    V v; double vv; int s = 0;
+    closest_to_origin(Y, n, MAXF, &v, &vv, &s);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+    *set = s;
+}
+
+void orc_barycentric(const double *pts, int n, double *out) {
+    V Y[4];
+    for (int k = 0; k < n; ++k) Y[k] = mk(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+    if (n == 2) bary_line(Y[0], Y[1], out, out + 1);
+    else if (n == 3) bary_plane(Y[0], Y[1], Y[2], out, out + 1, out + 2);
+    else bary_tetra(Y[0], Y[1], Y[2], Y[3], out, out + 1, out + 2, out + 3);
+}
+
+/* Brute-force closed-interval overlap count / listing (aabb_tree.py:465-500).
+ * a: (n,3,2), b: (m,3,2).  Pairs are written row-major (i outer) while
+ * k < cap; the total count is returned. */
+int64_t orc_all_aabbs_overlap(const double *a, int64_t n, const double *b, int64_t m,
+                              int64_t *pairs, int64_t cap) {
+    int64_t cnt = 0;
+    for (int64_t x = 0; x < n; ++x)
+        for (int64_t y = 0; y < m; ++y)
+            if (aabb_overlap(a + 6 * x, b + 6 * y)) {
+                if (cnt < cap) { pairs[2 * cnt] = x; pairs[2 * cnt + 1] = y; }
+                ++cnt;
+            }
+    return cnt;
+}
+
+/* Same predicate and set as orc_all_aabbs_overlap for large inputs: b is
+ * sorted by its x-min and each a row scans the window of b rows whose x-min
+ * is <= a's x-max.  order must hold b's indices sorted by b[.,0,0];
+ * sorted_xmin the matching x-min values; max_width the largest b x-extent.
+ * Output order: (a row, b index ascending in order). */
+int64_t orc_overlap_sorted(const double *a, int64_t n, const double *b, const int64_t *order,
+                           const double *sorted_xmin, int64_t m, double max_width,
+                           int64_t *pairs, int64_t cap) {
+    int64_t cnt = 0;
+    for (int64_t x = 0; x < n; ++x) {
+        const double *A = a + 6 * x;
+        /* b x-min must be >= A.xmin - max_width for b.xmax >= A.xmin to hold
+         * (conservative start), and <= A.xmax */
+        double lo_key = A[0] - max_width;
+        int64_t lo = 0, hi = m;
+        while (lo < hi) { int64_t mid = (lo + hi) / 2; if (sorted_xmin[mid] < lo_key) lo = mid + 1; else hi = mid; }
+        for (int64_t j = lo; j < m && sorted_xmin[j] <= A[1]; ++j) {
+            int64_t y = order[j];
+            if (aabb_overlap(A, b + 6 * y)) {
+                if (cnt < cap) { pairs[2 * cnt] = x; pairs[2 * cnt + 1] = y; }
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}

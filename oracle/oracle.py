@@ -1,0 +1,160 @@
+"""ctypes wrapper of the FP64 CPU oracle (d3d_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU
+baseline, never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libd3d_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.orc_num_threads.restype = ctypes.c_int
+        _lib.orc_all_aabbs_overlap.restype = ctypes.c_int64
+        _lib.orc_overlap_sorted.restype = ctypes.c_int64
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def set_threads(n):
+    lib().orc_set_threads(ctypes.c_int(int(n)))
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def _scene_args(scene):
+    f, i, vx, vy, vz, v64 = scene.oracle_args()
+    return [_p(f), _p(i), _p(vx), _p(vy), _p(vz), _p(v64)]
+
+
+def support(scene, shape_ids, dirs):
+    ids = np.ascontiguousarray(shape_ids, dtype=np.int32)
+    d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    out = np.empty((len(ids), 3))
+    lib().orc_support(*_scene_args(scene), _p(ids), _p(d), ctypes.c_int64(len(ids)), _p(out))
+    return out
+
+
+def aabbs(scene, shape_ids):
+    ids = np.ascontiguousarray(shape_ids, dtype=np.int32)
+    out = np.empty((len(ids), 3, 2))
+    lib().orc_aabbs(*_scene_args(scene), _p(ids), ctypes.c_int64(len(ids)), _p(out))
+    return out
+
+
+def gjk_distance(scene, pair_a, pair_b, tolerance=1e-10, max_distance_squared=100000.0,
+                 sanity_check=1e-8, max_iterations=128, want_simplex=False):
+    a = np.ascontiguousarray(pair_a, dtype=np.int32)
+    b = np.ascontiguousarray(pair_b, dtype=np.int32)
+    n = len(a)
+    dist = np.empty(n)
+    cpa = np.empty((n, 3))
+    cpb = np.empty((n, 3))
+    status = np.empty(n, dtype=np.int32)
+    iters = np.empty(n, dtype=np.int32)
+    simplex = np.empty((n, 3, 4, 3)) if want_simplex else None
+    npts = np.empty(n, dtype=np.int32) if want_simplex else None
+    lib().orc_gjk_distance(*_scene_args(scene), _p(a), _p(b), ctypes.c_int64(n),
+                           ctypes.c_double(tolerance), ctypes.c_double(max_distance_squared),
+                           ctypes.c_double(sanity_check), ctypes.c_int(max_iterations),
+                           _p(dist), _p(cpa), _p(cpb), _p(status), _p(iters), _p(simplex),
+                           _p(npts))
+    out = {"dist": dist, "cp_a": cpa, "cp_b": cpb, "status": status, "iterations": iters}
+    if want_simplex:
+        out["simplex"] = simplex
+        out["n_points"] = npts
+    return out
+
+
+def gjk_intersection(scene, pair_a, pair_b, tolerance=1e-10, max_iterations=128):
+    a = np.ascontiguousarray(pair_a, dtype=np.int32)
+    b = np.ascontiguousarray(pair_b, dtype=np.int32)
+    n = len(a)
+    hit = np.empty(n, dtype=np.int32)
+    iters = np.empty(n, dtype=np.int32)
+    lib().orc_gjk_intersection(*_scene_args(scene), _p(a), _p(b), ctypes.c_int64(n),
+                               ctypes.c_double(tolerance), ctypes.c_int(max_iterations),
+                               _p(hit), _p(iters))
+    return hit, iters
+
+
+def epa(scene, pair_a, pair_b, simplex, n_points, max_iterations=64, max_faces=128,
+        epsilon=1e-8):
+    a = np.ascontiguousarray(pair_a, dtype=np.int32)
+    b = np.ascontiguousarray(pair_b, dtype=np.int32)
+    n = len(a)
+    s = np.ascontiguousarray(simplex, dtype=np.float64).reshape(n, 36)
+    npts = np.ascontiguousarray(n_points, dtype=np.int32)
+    out = np.empty((n, 13))
+    status = np.empty(n, dtype=np.int32)
+    iters = np.empty(n, dtype=np.int32)
+    lib().orc_epa(*_scene_args(scene), _p(a), _p(b), ctypes.c_int64(n), _p(s), _p(npts),
+                  ctypes.c_int(max_iterations), ctypes.c_int(max_faces),
+                  ctypes.c_double(epsilon), _p(out), _p(status), _p(iters))
+    return {"depth": out[:, 0], "normal": out[:, 1:4], "mtv": out[:, 4:7],
+            "cp_a": out[:, 7:10], "cp_b": out[:, 10:13], "status": status,
+            "iterations": iters}
+
+
+def closest_point_simplex(points):
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(3)
+    s = ctypes.c_int32()
+    lib().orc_closest_point_simplex(_p(p), ctypes.c_int(len(p)), _p(out), ctypes.byref(s))
+    return out, s.value
+
+
+def barycentric(points):
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(len(p))
+    lib().orc_barycentric(_p(p), ctypes.c_int(len(p)), _p(out))
+    return out
+
+
+def all_aabbs_overlap(aabbs1, aabbs2, cap=None):
+    """Brute-force closed-interval pairs (i, j), row-major (i outer)."""
+    a = np.ascontiguousarray(aabbs1, dtype=np.float64).reshape(-1, 3, 2)
+    b = np.ascontiguousarray(aabbs2, dtype=np.float64).reshape(-1, 3, 2)
+    if cap is None:
+        cap = len(a) * len(b)
+    pairs = np.empty((max(cap, 1), 2), dtype=np.int64)
+    cnt = lib().orc_all_aabbs_overlap(_p(a), ctypes.c_int64(len(a)), _p(b),
+                                      ctypes.c_int64(len(b)), _p(pairs), ctypes.c_int64(cap))
+    return pairs[:min(cnt, cap)], cnt
+
+
+def overlap_pairs_sorted(aabbs1, aabbs2, cap):
+    """Same set as all_aabbs_overlap for large inputs (x-sorted sweep)."""
+    a = np.ascontiguousarray(aabbs1, dtype=np.float64).reshape(-1, 3, 2)
+    b = np.ascontiguousarray(aabbs2, dtype=np.float64).reshape(-1, 3, 2)
+    order = np.argsort(b[:, 0, 0], kind="stable").astype(np.int64)
+    xmin = np.ascontiguousarray(b[order, 0, 0])
+    width = float(np.max(b[:, 0, 1] - b[:, 0, 0])) * (1.0 + 1e-9) + 1e-300
+    pairs = np.empty((max(cap, 1), 2), dtype=np.int64)
+    cnt = lib().orc_overlap_sorted(_p(a), ctypes.c_int64(len(a)), _p(b), _p(order), _p(xmin),
+                                   ctypes.c_int64(len(b)), ctypes.c_double(width), _p(pairs),
+                                   ctypes.c_int64(cap))
+    return pairs[:min(cnt, cap)], cnt

@@ -1,0 +1,516 @@
+"""Pins the FP64 oracle (oracle/d3d_oracle.c) before it is trusted as the
+checker of the GPU path.
+
+Each test restates a known answer or invariant of the reference's own tests
+(file:line cited) or checks the oracle against an independent brute-force /
+closed-form computation (tests/geometry_brute.py), chosen so that a dropped
+term, a wrong sign or a transposed operand fails.  No GPU needed.
+"""
+import numpy as np
+import pytest
+from numpy.testing import assert_array_almost_equal
+from pytest import approx
+
+from distance3d_amd.scene import SceneBuilder
+from oracle import oracle
+from tests import geometry_brute as gb
+
+MAX_FLOAT = np.finfo(float).max
+
+
+def _scene(*adders):
+    b = SceneBuilder()
+    ids = [f(b) for f in adders]
+    return b.build(), ids
+
+
+def _dist(*adders, **kw):
+    s, ids = _scene(*adders)
+    r = oracle.gjk_distance(s, [ids[0]], [ids[1]], **kw)
+    return r["dist"][0], r["cp_a"][0], r["cp_b"][0], r["status"][0]
+
+
+def _hull(v):
+    return lambda b: b.add_hull(np.asarray(v, dtype=float))
+
+
+# --------------------------------------------------- reference known answers
+
+def test_points():  # test_gjk.py:7-18
+    p1, p2 = _hull([[0.0, 0.0, 0.0]]), _hull([[1.0, 0.0, 0.0]])
+    assert _dist(p1, p1)[0] == 0.0
+    assert _dist(p1, p2)[0] == 1.0
+
+
+def test_line_segments():  # test_gjk.py:21-53
+    s1 = _hull([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    p1, p2 = _hull([[0.0, 0.0, 0.0]]), _hull([[1.0, 0.0, 0.0]])
+    assert _dist(s1, s1)[0] == 0.0
+    assert _dist(s1, p1)[0] == 0.0
+    assert _dist(s1, p2)[0] == 0.0
+    assert _dist(s1, _hull([[0.0, 1.0, 0.0], [1.0, 1.0, 0.0]]))[0] == 1.0
+    assert _dist(s1, _hull([[0.5, -1.0, 0.0], [0.5, 1.0, 0.0]]))[0] == 0.0
+
+
+def test_boxes_touching():  # test_gjk.py:56-101
+    box = lambda b: b.add_box(np.eye(4), np.ones(3))
+    assert _dist(box, box)[0] == approx(0.0)
+    for dim1 in range(3):
+        for dim2 in range(3):
+            for dim3 in range(3):
+                for s1 in (-1, 1):
+                    for s2 in (-1, 1):
+                        for s3 in (-1, 1):
+                            T = np.eye(4)
+                            T[dim1, 3] = s1
+                            T[dim2, 3] = s2
+                            T[dim3, 3] = s3
+                            d = _dist(box, lambda b, T=T: b.add_box(T, np.ones(3)))[0]
+                            assert d == approx(0.0)
+
+
+def test_boxes_rotated_known_distance():  # test_gjk.py:104-126
+    T1 = np.array([[-0.29265666, -0.76990535, 0.56709596, 0.1867558],
+                   [0.93923897, -0.12018753, 0.32153556, -0.09772779],
+                   [-0.17939408, 0.62673815, 0.75829879, 0.09500884],
+                   [0., 0., 0., 1.]])
+    T2 = T1.copy()
+    T2[:3, 3] = [3.73511598, -1.95455576, 1.90017684]
+    d = _dist(lambda b: b.add_box(T1, np.array([2.89098828, 1.15032456, 2.37517511])),
+              lambda b: b.add_box(T2, np.array([0.96366276, 0.38344152, 0.79172504])))[0]
+    assert d == approx(1.7900192730149391)
+
+
+def test_spheres():  # test_gjk.py:132-187
+    s1 = lambda b: b.add_sphere(np.zeros(3), 1.0)
+    assert _dist(s1, s1)[0] == approx(0.0)
+    assert _dist(s1, lambda b: b.add_sphere(np.ones(3), 1.0))[0] == approx(0.0)
+    d, cp1, cp2, _ = _dist(s1, lambda b: b.add_sphere(np.array([0, 0, 3.0]), 1.0))
+    assert d == approx(1.0)
+    assert_array_almost_equal(cp1, [0, 0, 1])
+    assert_array_almost_equal(cp2, [0, 0, 2])
+
+
+def test_cylinders():  # test_gjk.py:190-248
+    c1 = lambda b: b.add_cylinder(np.eye(4), 1.0, 1.0)
+    assert _dist(c1, c1)[0] == approx(0.0)
+    T = np.eye(4)
+    T[:3, 3] = [3, 0, 0]
+    assert _dist(c1, lambda b: b.add_cylinder(T, 1.0, 1.0))[0] == approx(1.0)
+    T = np.eye(4)
+    T[:3, 3] = [0, 0, 4]
+    assert _dist(c1, lambda b: b.add_cylinder(T, 1.0, 1.0))[0] == approx(3.0)
+
+
+def test_capsules():  # test_gjk.py:251-309
+    c1 = lambda b: b.add_capsule(np.eye(4), 1.0, 1.0)
+    d, cp1, cp2, _ = _dist(c1, c1)
+    assert d == approx(0.0)
+    assert_array_almost_equal(cp1, cp2)
+    T = np.eye(4)
+    T[:3, 3] = [3, 0, 0]
+    d, cp1, cp2, _ = _dist(c1, lambda b: b.add_capsule(T, 1.0, 1.0))
+    assert d == approx(1.0)
+    assert_array_almost_equal(cp1, [1, 0, -0.5])
+    assert_array_almost_equal(cp2, [2, 0, -0.5])
+    T = np.eye(4)
+    T[:3, 3] = [0, 0, 4]
+    d, cp1, cp2, _ = _dist(c1, lambda b: b.add_capsule(T, 1.0, 1.0))
+    assert d == approx(1.0)
+    assert_array_almost_equal(cp1, [0, 0, 1.5])
+    assert_array_almost_equal(cp2, [0, 0, 2.5])
+
+
+def test_margin():  # test_gjk.py:476-482
+    box = lambda b: b.add_box(np.eye(4), np.array([2.0, 2.0, 2.0]))
+    boxm = lambda b: b.add_box(np.eye(4), np.array([2.0, 2.0, 2.0]), margin=0.1)
+    sph = lambda b: b.add_sphere(np.array([2.0, 0.0, 0.0]), 0.5)
+    assert _dist(box, sph)[0] == approx(_dist(boxm, sph)[0] + 0.1)
+
+
+def test_random_points_closest_points_in_boxes():  # test_gjk.py:311-338
+    rs = np.random.RandomState(23)
+    for k in range(100):
+        v1 = rs.rand(6, 3) * np.array([[2, 5, 1]])
+        v2 = rs.rand(6, 3) * (np.array([[1, 3, 1]]) if k < 50 else np.array([[-2, -3, 1]]))
+        d, cp1, cp2, _ = _dist(_hull(v1), _hull(v2))
+        assert 0 <= cp1[0] < 2 and 0 <= cp1[1] < 5 and 0 <= cp1[2] < 1
+        assert d == approx(np.linalg.norm(cp2 - cp1))
+        bd, _ = gb.hull_distance(v1, v2)
+        assert d == approx(bd, abs=1e-12)
+
+
+def test_too_far_away():  # test_gjk_jolt.py:88-95
+    d, cp1, cp2, st = _dist(_hull([[0.0, 0.0, 0.0]]), _hull([[100000000.0, 0.0, 0.0]]))
+    assert d == MAX_FLOAT
+    assert st == 3
+    assert np.all(np.isnan(cp1))
+
+
+# -------------------------------------------- closed forms and brute force
+
+def test_sphere_sphere_closed_form():
+    rs = np.random.RandomState(7)
+    for _ in range(200):
+        c1, c2 = rs.randn(3) * 2, rs.randn(3) * 2
+        r1, r2 = rs.uniform(0.1, 1), rs.uniform(0.1, 1)
+        d = _dist(lambda b: b.add_sphere(c1, r1), lambda b: b.add_sphere(c2, r2))[0]
+        expect = max(0.0, np.linalg.norm(c1 - c2) - r1 - r2)
+        if expect > 1e-6:
+            assert d == approx(expect, rel=1e-9)
+        elif np.linalg.norm(c1 - c2) - r1 - r2 < -1e-6:
+            assert d == 0.0
+
+
+def test_capsule_capsule_closed_form():
+    rs = np.random.RandomState(8)
+    for _ in range(200):
+        T1, T2 = gb.random_pose(rs, 1.5), gb.random_pose(rs, 1.5)
+        r1, r2 = rs.uniform(0.05, 0.5, 2)
+        h1, h2 = rs.uniform(0.2, 2, 2)
+        d = _dist(lambda b: b.add_capsule(T1, r1, h1), lambda b: b.add_capsule(T2, r2, h2))[0]
+        e1, e2 = T1[:3, 2] * h1 / 2, T2[:3, 2] * h2 / 2
+        sd, _, _ = gb.segment_segment(T1[:3, 3] - e1, T1[:3, 3] + e1, T2[:3, 3] - e2,
+                                      T2[:3, 3] + e2)
+        expect = sd - r1 - r2
+        if expect > 1e-6:
+            assert d == approx(expect, rel=1e-8)
+        elif expect < -1e-6:
+            assert d == 0.0
+
+
+def test_hull_hull_vs_minkowski_brute_force():
+    rs = np.random.RandomState(9)
+    n_sep = 0
+    for _ in range(150):
+        v1 = rs.randn(rs.randint(1, 9), 3) * 0.6 + rs.randn(3)
+        v2 = rs.randn(rs.randint(1, 9), 3) * 0.6 + rs.randn(3)
+        if len(v1) * len(v2) < 5:
+            continue
+        try:
+            bd, off = gb.hull_distance(v1, v2)
+        except Exception:  # flat Minkowski difference (Qhull refuses)
+            continue
+        d, cp1, cp2, st = _dist(_hull(v1), _hull(v2))
+        if bd > 1e-9:
+            n_sep += 1
+            assert d == approx(bd, rel=1e-9, abs=1e-12)
+            assert np.linalg.norm(cp1 - cp2) == approx(d, rel=1e-9, abs=1e-12)
+        elif off < -1e-9:
+            assert d == 0.0
+    assert n_sep > 30
+
+
+def test_posed_hull_equals_world_hull():
+    """A hull given in a local frame with a pose has the support of the
+    world-frame vertex set (the reference's ConvexHullVertices semantics)."""
+    rs = np.random.RandomState(10)
+    for _ in range(50):
+        T = gb.random_pose(rs)
+        local = rs.randn(12, 3).astype(np.float32).astype(float)
+        world = local @ T[:3, :3].T + T[:3, 3]
+        other = rs.randn(7, 3) + rs.randn(3) * 2
+        b = SceneBuilder()
+        i0 = b.add_hull(local, T[:3, :3], T[:3, 3])
+        i1 = b.add_hull(world)
+        i2 = b.add_hull(other)
+        s = b.build()
+        r = oracle.gjk_distance(s, [i0, i1], [i2, i2])
+        assert r["dist"][0] == approx(r["dist"][1], rel=1e-12, abs=1e-13)
+
+
+def test_simplex_subroutines_vs_subset_enumeration():  # test_gjk_jolt.py:11-85
+    rs = np.random.RandomState(0)
+    for n in (2, 3, 4):
+        for _ in range(200):
+            pts = rs.randn(n, 3)
+            cp, _ = oracle.closest_point_simplex(pts)
+            expect = gb.closest_point_of_points(pts)
+            assert_array_almost_equal(cp, expect, decimal=9)
+
+
+def test_barycentric_reconstructs_projection():  # test_gjk_jolt.py:11-50
+    rs = np.random.RandomState(1)
+    for _ in range(100):
+        a, b = rs.randn(3), rs.randn(3)
+        u, v = oracle.barycentric([a, b])
+        ab = b - a
+        expect = a - ab * (a @ ab) / (ab @ ab)  # origin projected on the line
+        assert_array_almost_equal(u * a + v * b, expect, decimal=10)
+        c = rs.randn(3)
+        u, v, w = oracle.barycentric([a, b, c])
+        n = np.cross(b - a, c - a)
+        expect = n * (a @ n) / (n @ n)  # origin projected on the plane
+        assert_array_almost_equal(u * a + v * b + w * c, expect, decimal=9)
+        assert u + v + w == approx(1.0)
+
+
+def test_intersection_flag_vs_minkowski_hull():  # test_gjk.py:341-354 pattern
+    rs = np.random.RandomState(84)
+    checked = 0
+    for _ in range(200):
+        v1 = rs.randn(rs.randint(4, 10), 3) * 0.7
+        v2 = rs.randn(rs.randint(4, 10), 3) * 0.7 + rs.randn(3) * 0.8
+        bd, off = gb.hull_distance(v1, v2)
+        if abs(off) < 1e-9 or (0 < bd < 1e-9):
+            continue  # inside the contact band
+        s, ids = _scene(_hull(v1), _hull(v2))
+        hit, _ = oracle.gjk_intersection(s, [0], [1])
+        dist = oracle.gjk_distance(s, [0], [1])["dist"][0]
+        truth = off <= 0
+        assert bool(hit[0]) == truth
+        assert (dist == 0.0) == truth
+        checked += 1
+    assert checked > 150
+
+
+# ------------------------------------------------------------ supports
+
+def test_box_support_known():  # test_geometry.py:12-27
+    b = SceneBuilder()
+    b.add_box(np.eye(4), np.array([2.0, 2.0, 2.0]))
+    s = b.build()
+    for d in np.eye(3):
+        p = oracle.support(s, [0], d[None])[0]
+        assert p @ d == approx(1.0)
+
+
+def test_cone_support_known():  # test_geometry.py:30-49
+    rs = np.random.RandomState(2323)
+    T = gb.random_pose(rs)
+    r, h = 0.7, 1.3
+    b = SceneBuilder()
+    b.add_cone(T, r, h)
+    s = b.build()
+    assert_array_almost_equal(oracle.support(s, [0], T[None, :3, 2])[0], T[:3, 3] + h * T[:3, 2])
+    assert_array_almost_equal(oracle.support(s, [0], T[None, :3, 1])[0], T[:3, 3] + r * T[:3, 1])
+    assert_array_almost_equal(oracle.support(s, [0], T[None, :3, 0])[0], T[:3, 3] + r * T[:3, 0])
+    assert_array_almost_equal(oracle.support(s, [0], np.zeros((1, 3)))[0], T[:3, 3])
+
+
+def test_disk_support_known():  # test_geometry.py:52-67
+    rs = np.random.RandomState(2223)
+    c, r = rs.randn(3), 0.8
+    n = rs.randn(3)
+    n /= np.linalg.norm(n)
+    b = SceneBuilder()
+    b.add_disk(c, r, n)
+    s = b.build()
+    from distance3d_amd.scene import plane_basis_from_normal
+    x, y = plane_basis_from_normal(n)
+    assert_array_almost_equal(oracle.support(s, [0], x[None])[0], c + r * x)
+    assert_array_almost_equal(oracle.support(s, [0], y[None])[0], c + r * y)
+    p = oracle.support(s, [0], n[None])[0]
+    assert (p - c) @ n == approx(0.0, abs=1e-12)
+    assert np.linalg.norm(p - c) == approx(r)
+    assert_array_almost_equal(oracle.support(s, [0], np.zeros((1, 3)))[0], c)
+
+
+def test_supports_are_maximisers():
+    """For every analytic shape, support(d) . d >= x . d for dense samples x
+    of the shape's surface (a wrong sign or axis fails this)."""
+    rs = np.random.RandomState(5)
+    for _ in range(20):
+        T = gb.random_pose(rs)
+        b = SceneBuilder()
+        b.add_sphere(T[:3, 3], 0.7)
+        b.add_capsule(T, 0.3, 1.2)
+        b.add_cylinder(T, 0.4, 0.9)
+        b.add_ellipsoid(T, np.array([0.3, 0.6, 0.9]))
+        b.add_box(T, np.array([0.3, 0.6, 0.9]))
+        b.add_cone(T, 0.5, 1.1)
+        b.add_disk(T[:3, 3], 0.6, T[:3, 2])
+        b.add_ellipse(T[:3, 3], T[:3, :2].T, np.array([0.4, 0.8]))
+        s = b.build()
+        # surface samples in local coordinates per shape
+        u = rs.randn(4000, 3)
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        ang = rs.uniform(0, 2 * np.pi, 4000)
+        z = rs.uniform(-1, 1, 4000)
+        ring = np.column_stack((np.cos(ang), np.sin(ang), np.zeros(4000)))
+        samples = [
+            u * 0.7,
+            np.vstack((u * 0.3 + [0, 0, 0.6], u * 0.3 - [0, 0, 0.6])),
+            ring * 0.4 + np.outer(z, [0, 0, 0.45]),
+            u * [0.3, 0.6, 0.9],
+            (np.sign(rs.randn(4000, 3)) * 0.5) * [0.3, 0.6, 0.9],
+            np.vstack((ring * 0.5, [[0, 0, 1.1]])),
+            ring * 0.6,
+            ring * [0.4, 0.8, 0],
+        ]
+        for k, smp in enumerate(samples):
+            world = smp @ T[:3, :3].T + T[:3, 3]
+            for _ in range(5):
+                d = rs.randn(3)
+                p = oracle.support(s, [k], d[None])[0]
+                assert p @ d >= np.max(world @ d) - 1e-9
+
+
+# --------------------------------------------------------------- AABBs
+
+def test_aabb_known_answers():  # test_colliders.py:35-93
+    b = SceneBuilder()
+    b.add_hull(np.array([[0.0, 0.0, 1.0], [0.0, 1.0, 0.0]]))
+    b.add_box(np.eye(4), np.ones(3))
+    b.add_cone(np.eye(4), 0.5, 1.0)
+    T = np.eye(4)
+    T[:3, 3] = 0.1, 0.2, 0.3
+    b.add_cone(T, 0.5, 1.0)
+    b.add_disk(np.zeros(3), 0.5, np.array([0.0, 0.0, 1.0]))
+    b.add_disk(np.array([-0.1, -0.2, -0.3]), 0.5, np.array([0.0, 0.0, 1.0]))
+    b.add_box(np.eye(4), np.ones(3), margin=0.1)
+    s = b.build()
+    out = oracle.aabbs(s, np.arange(7))
+    assert_array_almost_equal(out[0], [[0, 0], [0, 1], [0, 1]])
+    assert_array_almost_equal(out[1], [[-0.5, 0.5]] * 3)
+    assert_array_almost_equal(out[2], [[-0.5, 0.5], [-0.5, 0.5], [0.0, 1.0]])
+    assert_array_almost_equal(out[3], [[-0.4, 0.6], [-0.3, 0.7], [0.3, 1.3]])
+    assert_array_almost_equal(out[4], [[-0.5, 0.5], [-0.5, 0.5], [0.0, 0.0]])
+    assert_array_almost_equal(out[5], [[-0.6, 0.4], [-0.7, 0.3], [-0.3, -0.3]])
+    assert_array_almost_equal(out[6], [[-0.6, 0.6]] * 3)
+
+
+def test_aabb_equals_support_extents():
+    """Box, sphere, capsule, cylinder, disk and cone AABBs are tight: they
+    equal the support values along the six axis directions."""
+    rs = np.random.RandomState(11)
+    for _ in range(30):
+        T = gb.random_pose(rs)
+        b = SceneBuilder()
+        b.add_box(T, rs.uniform(0.1, 2, 3))
+        b.add_sphere(T[:3, 3], 0.6)
+        b.add_capsule(T, 0.3, 1.4)
+        b.add_cylinder(T, 0.4, 0.8)
+        b.add_disk(T[:3, 3], 0.5, T[:3, 2])
+        b.add_cone(T, 0.5, 1.5)
+        s = b.build()
+        box = oracle.aabbs(s, np.arange(6))
+        for k in range(6):
+            for a in range(3):
+                e = np.zeros(3)
+                e[a] = 1
+                hi = oracle.support(s, [k], e[None])[0][a]
+                lo = oracle.support(s, [k], -e[None])[0][a]
+                assert box[k, a, 1] == approx(hi, abs=1e-9)
+                assert box[k, a, 0] == approx(lo, abs=1e-9)
+
+
+# ---------------------------------------------------------- broad phase
+
+def test_all_aabbs_overlap_known():  # test_aabb_tree.py:6-55
+    aabbs = np.array([[[0, 2], [0, 2], [0, 1]], [[1, 3], [1, 3], [0, 1]],
+                      [[3, 4], [0, 1], [0, 1]]], dtype=float)
+    aabbs2 = np.array([[[1.5, 3.5], [0, 0.5], [0, 1]], [[1.5, 2], [2.5, 4], [0, 1]],
+                       [[3.5, 4.5], [0, 0.5], [0, 1]]], dtype=float)
+    pairs, cnt = oracle.all_aabbs_overlap(aabbs, aabbs2)
+    by_query = {j: sorted(pairs[pairs[:, 1] == j, 0].tolist()) for j in range(3)}
+    assert by_query[0] == [0, 2]
+    assert by_query[1] == [1]
+    assert by_query[2] == [2]
+    assert sorted(np.unique(pairs[:, 0])) == [0, 1, 2]
+    assert sorted(np.unique(pairs[:, 1])) == [0, 1, 2]
+    a1 = np.array([[0, 10], [0, 10], [0, 10]], dtype=float)
+    a2 = np.array([[8, 12], [3, 4], [4, 5]], dtype=float)
+    a3 = np.array([[20, 30], [0, 1], [0, 2]], dtype=float)
+    a4 = np.array([[-50, 50], [-50, 50], [-50, 50]], dtype=float)
+    pairs, _ = oracle.all_aabbs_overlap(np.array([a1, a2, a3]), np.array([a1, a4]))
+    assert sorted(pairs[pairs[:, 1] == 0, 0].tolist()) == [0, 1]
+    assert sorted(pairs[pairs[:, 1] == 1, 0].tolist()) == [0, 1, 2]
+
+
+def test_overlap_touching_faces_count():
+    """Closed intervals: a 3x3x3 grid of unit cubes touching face to face
+    gives every cube all neighbours within Chebyshev distance 1."""
+    g = np.array([[x, y, z] for x in range(3) for y in range(3) for z in range(3)], float)
+    boxes = np.stack([np.stack([g[:, k], g[:, k] + 1], axis=1) for k in range(3)], axis=1)
+    pairs, cnt = oracle.all_aabbs_overlap(boxes, boxes)
+    cheb = np.max(np.abs(g[:, None] - g[None]), axis=2)
+    assert cnt == int(np.sum(cheb <= 1))
+
+
+def test_sorted_sweep_equals_brute_force():
+    rs = np.random.RandomState(12)
+    lo = rs.uniform(0, 10, (700, 3))
+    a = np.stack([lo, lo + rs.uniform(0.01, 1.0, (700, 3))], axis=2)
+    lo2 = rs.uniform(0, 10, (300, 3))
+    b = np.stack([lo2, lo2 + rs.uniform(0.01, 2.0, (300, 3))], axis=2)
+    # include exact touches
+    b[:50, 0, 0] = a[:50, 0, 1]
+    p1, c1 = oracle.all_aabbs_overlap(b, a)
+    p2, c2 = oracle.overlap_pairs_sorted(b, a, cap=c1 + 10)
+    assert c1 == c2
+    assert set(map(tuple, p1)) == set(map(tuple, p2))
+
+
+# ------------------------------------------------------------------ EPA
+
+def _epa_from(v1, v2, **kw):
+    s, ids = _scene(_hull(v1), _hull(v2))
+    g = oracle.gjk_distance(s, [0], [1], want_simplex=True)
+    return g, oracle.epa(s, [0], [1], g["simplex"], g["n_points"], **kw)
+
+
+def test_epa_known_mtv():  # test_epa.py:7-34
+    v1 = np.array([[1.76405235, 0.40015721, 0.97873798], [2.2408932, 1.86755799, -0.97727788],
+                   [0.4105985, 0.14404357, 1.45427351], [0.33367433, 1.49407907, -0.20515826],
+                   [0.3130677, -0.85409574, -2.55298982], [2.26975462, -1.45436567, 0.04575852],
+                   [-0.18718385, 1.53277921, 1.46935877]])
+    v2 = np.array([[-2.32605299, -0.31242692, 0.07599278], [0.88503416, 1.23786508, -0.467683],
+                   [-0.64755927, -1.01306774, -1.50037412], [-2.05152671, 1.98626062, -0.59000837],
+                   [-0.78333082, -1.21731013, 0.69713417], [-1.95915437, -0.17725505, -0.97582275],
+                   [0.04164598, -0.47531991, -1.26098837], [-0.37343875, 0.4638171, -0.01383896],
+                   [-0.04278462, -0.59883687, -0.44309735]])
+    g, e = _epa_from(v1, v2)
+    assert g["dist"][0] == 0.0
+    assert e["status"][0] == 1
+    assert_array_almost_equal(e["mtv"][0], [-0.387287, 0.179576, -0.176204])
+    assert_array_almost_equal(e["cp_a"][0] - e["cp_b"][0], e["mtv"][0])
+
+
+def test_epa_separation_invariant():  # test_epa.py:37-60
+    rs = np.random.RandomState(25)
+    done = 0
+    for _ in range(50):
+        v1 = rs.rand(6, 3) * 2
+        v2 = rs.rand(6, 3)
+        g, e = _epa_from(v1, v2)
+        if g["dist"][0] > 0.0:
+            continue
+        assert e["status"][0] == 1
+        depth = np.linalg.norm(e["mtv"][0])
+        pdir = e["mtv"][0] / depth
+        moved = v2 + (1.0 + depth) * pdir[None]
+        s, _ = _scene(_hull(v1), _hull(moved))
+        assert oracle.gjk_distance(s, [0], [1])["dist"][0] == approx(1.0)
+        done += 1
+    assert done > 10
+
+
+def test_epa_depth_vs_minkowski_brute_force():
+    rs = np.random.RandomState(26)
+    done = 0
+    for _ in range(80):
+        v1 = rs.randn(rs.randint(5, 14), 3)
+        v2 = rs.randn(rs.randint(5, 14), 3) * 0.8 + rs.randn(3) * 0.5
+        g, e = _epa_from(v1, v2)
+        if g["dist"][0] > 0.0 or e["status"][0] != 1:
+            continue
+        depth, normal = gb.penetration_depth(v1, v2)
+        assert e["depth"][0] == approx(depth, rel=1e-7, abs=1e-9)
+        assert_array_almost_equal(e["cp_a"][0] - e["cp_b"][0], e["mtv"][0], decimal=9)
+        done += 1
+    assert done > 40
+
+
+def test_epa_seed_from_short_simplex():
+    """Identical boxes stop GJK with fewer than four points; the blown-up
+    seed still yields the true depth (half the box size along some axis)."""
+    b = SceneBuilder()
+    b.add_box(np.eye(4), np.ones(3))
+    b.add_box(np.eye(4), np.ones(3))
+    s = b.build()
+    g = oracle.gjk_distance(s, [0], [1], want_simplex=True)
+    assert g["dist"][0] == 0.0
+    e = oracle.epa(s, [0], [1], g["simplex"], g["n_points"])
+    assert e["status"][0] == 1
+    assert e["depth"][0] == approx(1.0)
