@@ -67,6 +67,7 @@ _SIGNATURES = {
     "d3d_gjk_distance_batch": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params),
                                _P, _P, _P, _P, _P, _P, _P, _P],
     "d3d_gjk_intersection_batch": [_SCENE, _P, _P, _I64, _D, _I32, _P, _P, _P],
+    "d3d_epa_batch": [_SCENE, _P, _P, _I64, _P, _P, _I32, _I32, _D, _P, _P, _P, _P],
 }
 
 _lock = threading.Lock()

@@ -90,6 +90,45 @@ __device__ __forceinline__ Shape load_shape(const d3d_scene &s, int k) {
     return h;
 }
 
+__device__ __forceinline__ Shape empty_shape() {
+    Shape h;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) h.R[j] = 0.0;
+    h.R[0] = h.R[4] = h.R[8] = 1.0;
+    h.t[0] = h.t[1] = h.t[2] = 0.0;
+    h.p[0] = h.p[1] = h.p[2] = 0.0;
+    h.margin = 0.0;
+    h.type = D3D_SPHERE;
+    h.nv = 0;
+    h.voff = 0;
+    h.voff64 = -1;
+    return h;
+}
+
+// Index and vertex-range checks so a malformed table cannot make a lane read
+// outside the pools.
+__device__ __forceinline__ bool load_checked(const d3d_scene &s, int k, Shape &h) {
+    if (k < 0 || (int64_t)k >= s.n_shapes) {
+        h = empty_shape();
+        return false;
+    }
+    h = load_shape(s, k);
+    if (h.type < D3D_HULL || h.type > D3D_ELLIPSE) {
+        h = empty_shape();
+        return false;
+    }
+    if (h.type == D3D_HULL) {
+        const bool ok32 = h.nv >= 1 && h.voff >= 0 && (int64_t)h.voff + h.nv <= s.n_vertices;
+        const bool ok64 =
+            h.voff64 < 0 || ((int64_t)h.voff64 + h.nv <= s.n_vertices64 && s.v64 != nullptr);
+        if (!ok32 || !ok64) {
+            h = empty_shape();
+            return false;
+        }
+    }
+    return true;
+}
+
 // R^T d, i.e. np.dot(A2B[:3, :3].T, d) (geometry.py:202)
 __device__ __forceinline__ v3 to_local(const Shape &h, v3 d) {
     return mk(h.R[0] * d.x + h.R[3] * d.y + h.R[6] * d.z,

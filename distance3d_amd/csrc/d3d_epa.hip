@@ -1,0 +1,487 @@
+// d3d_epa.hip -- batched EPA penetration depth with contact points (gfx950).
+//
+// One wavefront (one 64-thread workgroup) per pair.  The polytope lives in
+// LDS: vertex records (Minkowski point and the support points on A and B) and
+// up to 128 faces (vertex indices, unit normal, distance); lane l owns faces l
+// and l + 64 in every face-parallel step.  Each iteration:
+//   closest face    wavefront argmin over face distances (first index on ties)
+//   support         both colliders along the face normal (hull sweeps use all
+//                   64 lanes, see coop_hull_argmax)
+//   visible faces   per-face test n . (w - v0) > eps
+//   horizon         edges of removed faces whose reverse edge is not on
+//                   another removed face (vertex-index matching)
+//   rebuild         surviving faces compacted in order, new faces (horizon
+//                   edge, w) appended in horizon order, via ballot prefixes
+// The algorithm and its ordering rules are those of the FP64 checker's EPA
+// (epa.py:9-202 with an oriented seed, index-matched horizon edges, contact
+// points and a face cap that ends the loop instead of raising).
+#include "d3d_device.hpp"
+#include "d3d_runtime.hpp"
+
+namespace d3d {
+
+constexpr int kEpaMaxFaces = 128;
+constexpr int kEpaMaxIter = 64;
+constexpr int kEpaMaxVerts = kEpaMaxIter + 8;
+constexpr int kEpaBlock = 64;
+
+struct EpaLds {
+    double wx[kEpaMaxVerts], wy[kEpaMaxVerts], wz[kEpaMaxVerts];
+    double ax[kEpaMaxVerts], ay[kEpaMaxVerts], az[kEpaMaxVerts];
+    double bx[kEpaMaxVerts], by[kEpaMaxVerts], bz[kEpaMaxVerts];
+    double fnx[kEpaMaxFaces], fny[kEpaMaxFaces], fnz[kEpaMaxFaces], fd[kEpaMaxFaces];
+    int fv0[kEpaMaxFaces], fv1[kEpaMaxFaces], fv2[kEpaMaxFaces];
+    int rv0[kEpaMaxFaces], rv1[kEpaMaxFaces], rv2[kEpaMaxFaces];
+};
+
+// Support of a wave-uniform collider along a wave-uniform direction.
+__device__ __forceinline__ v3 support_uniform(const d3d_scene &s, const Shape &h, v3 d) {
+    v3 out;
+    if (h.type == D3D_HULL) {
+        const v3 l = to_local(h, d);
+        const int idx = coop_hull_argmax(s, h.voff, h.voff64, h.nv, h.p[0], l);
+        out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));
+    } else {
+        out = support_analytic(h, d);
+    }
+    return apply_margin(h, out, d);
+}
+
+__device__ __forceinline__ v3 lds_w(const EpaLds &L, int v) { return mk(L.wx[v], L.wy[v], L.wz[v]); }
+
+__device__ __forceinline__ int wave_excl_scan(int c, int lane, int &total) {
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+    }
+    total = __shfl(incl, kWave - 1);
+    return incl - c;
+}
+
+// make_face of the checker: unit normal of (w1 - w0) x (w2 - w0) and the
+// distance w0 . n; false when the normal has zero length.
+__device__ __forceinline__ bool face_normal(v3 w0, v3 w1, v3 w2, v3 &n, double &dist) {
+    const v3 c = cross(w1 - w0, w2 - w0);
+    const double l = sqrt(dot(c, c));
+    if (!(l > 0.0)) return false;
+    n = mk(c.x / l, c.y / l, c.z / l);
+    dist = dot(w0, n);
+    return true;
+}
+
+struct SeedVerts {
+    v3 W[4], A[4], B[4];
+};
+
+// Seed tetrahedron containing the origin from a GJK simplex of n points
+// (uniform across the wavefront).  Returns false for a degenerate seed.
+__device__ bool epa_seed(const d3d_scene &s, const Shape &SA, const Shape &SB, SeedVerts &V, int n) {
+    if (n < 1 || n > 4) return false;
+    if (n == 1) {
+        int best = -1;
+        double bl = 0.0;
+        v3 bw = V.W[0], bpa = V.A[0], bpb = V.B[0];
+        for (int k = 0; k < 6; ++k) {
+            const double sg = (k & 1) ? -1.0 : 1.0;
+            const v3 d = (k < 2) ? mk(sg, 0.0, 0.0) : ((k < 4) ? mk(0.0, sg, 0.0) : mk(0.0, 0.0, sg));
+            const v3 pa = support_uniform(s, SA, d), pb = support_uniform(s, SB, -d);
+            const v3 w = pa - pb;
+            const v3 e = w - V.W[0];
+            const double l = dot(e, e);
+            if (l > bl) {
+                bl = l;
+                best = k;
+                bw = w;
+                bpa = pa;
+                bpb = pb;
+            }
+        }
+        if (best < 0 || !(bl > kEps)) return false;
+        V.W[1] = bw;
+        V.A[1] = bpa;
+        V.B[1] = bpb;
+        n = 2;
+    }
+    if (n == 2) {
+        const v3 ab = V.W[1] - V.W[0];
+        const double ax = fabs(ab.x), ay = fabs(ab.y), az = fabs(ab.z);
+        const v3 e = (ax <= ay && ax <= az) ? mk(1, 0, 0) : ((ay <= az) ? mk(0, 1, 0) : mk(0, 0, 1));
+        const v3 p1 = cross(ab, e);
+        const v3 p2 = cross(ab, p1);
+        int best = -1;
+        double bl = 0.0;
+        v3 bw = V.W[0], bpa = V.A[0], bpb = V.B[0];
+        for (int k = 0; k < 4; ++k) {
+            const v3 c = (k == 0) ? p1 : ((k == 1) ? -p1 : ((k == 2) ? p2 : -p2));
+            const v3 pa = support_uniform(s, SA, c), pb = support_uniform(s, SB, -c);
+            const v3 w = pa - pb;
+            const v3 cr = cross(ab, w - V.W[0]);
+            const double l = dot(cr, cr);
+            if (l > bl) {
+                bl = l;
+                best = k;
+                bw = w;
+                bpa = pa;
+                bpb = pb;
+            }
+        }
+        if (best < 0 || !(bl > kEps * dot(ab, ab))) return false;
+        V.W[2] = bw;
+        V.A[2] = bpa;
+        V.B[2] = bpb;
+        n = 3;
+    }
+    if (n == 3) {
+        const v3 nrm = cross(V.W[1] - V.W[0], V.W[2] - V.W[0]);
+        if (!(dot(nrm, nrm) > 0.0)) return false;
+        const v3 wp_pa = support_uniform(s, SA, nrm), wp_pb = support_uniform(s, SB, -nrm);
+        const v3 wp = wp_pa - wp_pb;
+        const v3 wm_pa = support_uniform(s, SA, -nrm), wm_pb = support_uniform(s, SB, nrm);
+        const v3 wm = wm_pa - wm_pb;
+        const double hp = dot(wp - V.W[0], nrm), hm = -dot(wm - V.W[0], nrm);
+        if (hp >= hm) {
+            V.W[3] = wp;
+            V.A[3] = wp_pa;
+            V.B[3] = wp_pb;
+        } else {
+            V.W[3] = wm;
+            V.A[3] = wm_pa;
+            V.B[3] = wm_pb;
+        }
+        if (!((hp > hm ? hp : hm) > 0.0)) return false;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(kEpaBlock) void epa_kernel(
+    d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, int64_t n,
+    const int64_t *__restrict__ n_dev, const int32_t *__restrict__ work,
+    const double *__restrict__ simplex, const int32_t *__restrict__ npts, int max_iter,
+    int max_faces, double eps, double *__restrict__ out, int32_t *__restrict__ status_out,
+    int32_t *__restrict__ iters_out) {
+    __shared__ EpaLds L;
+    const int lane = threadIdx.x;
+    const int64_t count = n_dev ? *n_dev : n;
+    for (int64_t item = blockIdx.x; item < count; item += gridDim.x) {
+        const int64_t pid = work ? (int64_t)work[item] : item;
+        Shape SA, SB;
+        const bool okA = load_checked(s, pa[pid], SA);
+        const bool okB = load_checked(s, pb[pid], SB);
+        double *o = out + 13 * pid;
+        if (!(okA && okB)) {
+            if (lane < 13) o[lane] = NAN;
+            if (lane == 0) {
+                status_out[pid] = 0x200;
+                if (iters_out) iters_out[pid] = 0;
+            }
+            continue;
+        }
+
+        // ---- seed
+        SeedVerts V;
+        const double *S = simplex + 36 * pid;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            V.W[r] = mk(S[3 * r], S[3 * r + 1], S[3 * r + 2]);
+            V.A[r] = mk(S[12 + 3 * r], S[13 + 3 * r], S[14 + 3 * r]);
+            V.B[r] = mk(S[24 + 3 * r], S[25 + 3 * r], S[26 + 3 * r]);
+        }
+        int status = 0;
+        bool seed_ok = epa_seed(s, SA, SB, V, npts[pid]);
+        int nf = 0;
+        if (seed_ok) {
+            const v3 nrm = cross(V.W[1] - V.W[0], V.W[2] - V.W[0]);
+            if (dot(V.W[3] - V.W[0], nrm) > 0.0) {
+                v3 t = V.W[1]; V.W[1] = V.W[2]; V.W[2] = t;
+                t = V.A[1]; V.A[1] = V.A[2]; V.A[2] = t;
+                t = V.B[1]; V.B[1] = V.B[2]; V.B[2] = t;
+            }
+            if (lane < 4) {
+                const v3 w = (lane == 0) ? V.W[0] : (lane == 1) ? V.W[1] : (lane == 2) ? V.W[2] : V.W[3];
+                const v3 a = (lane == 0) ? V.A[0] : (lane == 1) ? V.A[1] : (lane == 2) ? V.A[2] : V.A[3];
+                const v3 b = (lane == 0) ? V.B[0] : (lane == 1) ? V.B[1] : (lane == 2) ? V.B[2] : V.B[3];
+                L.wx[lane] = w.x; L.wy[lane] = w.y; L.wz[lane] = w.z;
+                L.ax[lane] = a.x; L.ay[lane] = a.y; L.az[lane] = a.z;
+                L.bx[lane] = b.x; L.by[lane] = b.y; L.bz[lane] = b.z;
+            }
+            // seed faces {0,1,2}, {0,3,1}, {0,2,3}, {1,3,2}; a degenerate one
+            // makes the seed degenerate
+            bool fok = true;
+            int i0 = 0, i1 = 0, i2 = 0;
+            if (lane < 4) {
+                i0 = (lane == 3) ? 1 : 0;
+                i1 = (lane == 0) ? 1 : (lane == 1) ? 3 : (lane == 2) ? 2 : 3;
+                i2 = (lane == 0) ? 2 : (lane == 1) ? 1 : (lane == 2) ? 3 : 2;
+                const v3 w0 = (i0 == 0) ? V.W[0] : V.W[1];
+                const v3 w1 = (i1 == 1) ? V.W[1] : (i1 == 2) ? V.W[2] : V.W[3];
+                const v3 w2 = (i2 == 1) ? V.W[1] : (i2 == 2) ? V.W[2] : V.W[3];
+                v3 nn;
+                double dd;
+                fok = face_normal(w0, w1, w2, nn, dd);
+                L.fv0[lane] = i0; L.fv1[lane] = i1; L.fv2[lane] = i2;
+                L.fnx[lane] = nn.x; L.fny[lane] = nn.y; L.fnz[lane] = nn.z; L.fd[lane] = dd;
+            }
+            if (__ballot(lane < 4 && !fok) != 0ull) seed_ok = false;
+            nf = 4;
+        }
+        __syncthreads();
+        if (!seed_ok) {
+            if (lane < 13) o[lane] = 0.0;
+            if (lane == 0) {
+                status_out[pid] = 3;
+                if (iters_out) iters_out[pid] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+
+        int nv = 4;
+        int it = 0;
+        int kb = 0;
+        double depth = 0.0;
+        v3 normal = mk(0.0, 0.0, 0.0);
+        for (it = 0; it < max_iter; ++it) {
+            // closest face: smallest distance, first index on ties
+            double bd = INFINITY;
+            int bk = 0x7fffffff;
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int k = sl * kWave + lane;
+                if (k < nf) {
+                    const double d = L.fd[k];
+                    if (d < bd) { bd = d; bk = k; }
+                }
+            }
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const double od = __shfl_xor(bd, off);
+                const int ok = __shfl_xor(bk, off);
+                const bool take = (od < bd) || (od == bd && ok < bk);
+                bd = take ? od : bd;
+                bk = take ? ok : bk;
+            }
+            kb = bk;
+            const v3 cn = mk(L.fnx[kb], L.fny[kb], L.fnz[kb]);
+            const double cdist = L.fd[kb];
+            const v3 spa = support_uniform(s, SA, cn), spb = support_uniform(s, SB, -cn);
+            const v3 w = spa - spb;
+            const double wn = dot(w, cn);
+            if (wn - cdist < eps) {
+                status = 1;
+                depth = wn;
+                normal = cn;
+                break;
+            }
+            if (nv >= kEpaMaxVerts) {
+                status = 2;
+                break;
+            }
+            const int wi = nv;
+            if (lane == 0) {
+                L.wx[wi] = w.x; L.wy[wi] = w.y; L.wz[wi] = w.z;
+                L.ax[wi] = spa.x; L.ay[wi] = spa.y; L.az[wi] = spa.z;
+                L.bx[wi] = spb.x; L.by[wi] = spb.y; L.bz[wi] = spb.z;
+            }
+            ++nv;
+            // faces that see w
+            bool rem[2];
+            int fv[2][3];
+            v3 fn[2];
+            double fdist[2];
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int k = sl * kWave + lane;
+                rem[sl] = false;
+                fv[sl][0] = fv[sl][1] = fv[sl][2] = 0;
+                fn[sl] = mk(0.0, 0.0, 0.0);
+                fdist[sl] = 0.0;
+                if (k < nf) {
+                    fv[sl][0] = L.fv0[k]; fv[sl][1] = L.fv1[k]; fv[sl][2] = L.fv2[k];
+                    fn[sl] = mk(L.fnx[k], L.fny[k], L.fnz[k]);
+                    fdist[sl] = L.fd[k];
+                    rem[sl] = dot(fn[sl], w - lds_w(L, fv[sl][0])) > eps;
+                }
+            }
+            const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+            const unsigned long long m0 = __ballot(rem[0]), m1 = __ballot(rem[1]);
+            const int r0 = __popcll(m0);
+            const int nrem = r0 + __popcll(m1);
+            const int rrank0 = __popcll(m0 & lt), rrank1 = r0 + __popcll(m1 & lt);
+            if (rem[0]) { L.rv0[rrank0] = fv[0][0]; L.rv1[rrank0] = fv[0][1]; L.rv2[rrank0] = fv[0][2]; }
+            if (rem[1]) { L.rv0[rrank1] = fv[1][0]; L.rv1[rrank1] = fv[1][1]; L.rv2[rrank1] = fv[1][2]; }
+            __syncthreads();
+            // horizon edges of my removed faces (hz) and whether their new
+            // faces are non-degenerate (hv), in (removed order, edge 0..2)
+            // order
+            v3 hn[2][3];
+            double hd[2][3];
+            bool hz[2][3], hv[2][3];
+            int cnt[2] = {0, 0};
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int myrank = sl == 0 ? rrank0 : rrank1;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    hz[sl][e] = hv[sl][e] = false;
+                    hn[sl][e] = mk(0.0, 0.0, 0.0);
+                    hd[sl][e] = 0.0;
+                    if (!rem[sl]) continue;
+                    const int a = fv[sl][e], b = fv[sl][(e + 1) % 3];
+                    bool shared = false;
+                    for (int m = 0; m < nrem && !shared; ++m) {
+                        if (m == myrank) continue;
+                        const int q0 = L.rv0[m], q1 = L.rv1[m], q2 = L.rv2[m];
+                        shared = (q0 == b && q1 == a) || (q1 == b && q2 == a) || (q2 == b && q0 == a);
+                    }
+                    if (shared) continue;
+                    hz[sl][e] = true;
+                    hv[sl][e] = face_normal(lds_w(L, a), lds_w(L, b), w, hn[sl][e], hd[sl][e]);
+                    cnt[sl] += hv[sl][e] ? 1 : 0;
+                }
+            }
+            int tot0, tot1;
+            const int pre0 = wave_excl_scan(cnt[0], lane, tot0);
+            const int pre1 = tot0 + wave_excl_scan(cnt[1], lane, tot1);
+            const bool kept0 = (lane < nf) && !rem[0];
+            const bool kept1 = (kWave + lane < nf) && !rem[1];
+            const unsigned long long k0 = __ballot(kept0), k1 = __ballot(kept1);
+            const int nkept = __popcll(k0) + __popcll(k1);
+            const int kpos0 = __popcll(k0 & lt), kpos1 = __popcll(k0) + __popcll(k1 & lt);
+            // face cap: the checker appends new faces edge by edge and stops
+            // at the first horizon edge (degenerate or not) met with
+            // n_kept + (valid new faces so far) >= max_faces
+            bool cap = false;
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                int before = sl == 0 ? pre0 : pre1;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    if (!hz[sl][e]) continue;
+                    if (nkept + before >= max_faces) cap = true;
+                    before += hv[sl][e] ? 1 : 0;
+                }
+            }
+            cap = __ballot(cap) != 0ull;
+            __syncthreads();  // all reads of the old face list are done
+            if (kept0) {
+                L.fv0[kpos0] = fv[0][0]; L.fv1[kpos0] = fv[0][1]; L.fv2[kpos0] = fv[0][2];
+                L.fnx[kpos0] = fn[0].x; L.fny[kpos0] = fn[0].y; L.fnz[kpos0] = fn[0].z;
+                L.fd[kpos0] = fdist[0];
+            }
+            if (kept1) {
+                L.fv0[kpos1] = fv[1][0]; L.fv1[kpos1] = fv[1][1]; L.fv2[kpos1] = fv[1][2];
+                L.fnx[kpos1] = fn[1].x; L.fny[kpos1] = fn[1].y; L.fnz[kpos1] = fn[1].z;
+                L.fd[kpos1] = fdist[1];
+            }
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                int pos = nkept + (sl == 0 ? pre0 : pre1);
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    if (!hv[sl][e]) continue;
+                    if (pos < max_faces) {
+                        L.fv0[pos] = fv[sl][e];
+                        L.fv1[pos] = fv[sl][(e + 1) % 3];
+                        L.fv2[pos] = wi;
+                        L.fnx[pos] = hn[sl][e].x; L.fny[pos] = hn[sl][e].y; L.fnz[pos] = hn[sl][e].z;
+                        L.fd[pos] = hd[sl][e];
+                    }
+                    ++pos;
+                }
+            }
+            const int added = tot0 + tot1;
+            nf = nkept + (added < max_faces - nkept ? added : (max_faces - nkept > 0 ? max_faces - nkept : 0));
+            __syncthreads();
+            if (cap || nf == 0) {
+                status = 2;
+                break;
+            }
+        }
+        if (status != 1) {
+            double bd = INFINITY;
+            int bk = 0x7fffffff;
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int k = sl * kWave + lane;
+                if (k < nf) {
+                    const double d = L.fd[k];
+                    if (d < bd) { bd = d; bk = k; }
+                }
+            }
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const double od = __shfl_xor(bd, off);
+                const int ok = __shfl_xor(bk, off);
+                const bool take = (od < bd) || (od == bd && ok < bk);
+                bd = take ? od : bd;
+                bk = take ? ok : bk;
+            }
+            kb = bk == 0x7fffffff ? 0 : bk;
+            if (nf > 0) {
+                depth = L.fd[kb];
+                normal = mk(L.fnx[kb], L.fny[kb], L.fnz[kb]);
+            }
+        }
+        const int iterations = it + (status == 1 ? 1 : 0);
+        v3 ca = mk(0.0, 0.0, 0.0), cb = mk(0.0, 0.0, 0.0);
+        if (nf > 0) {
+            const int i0 = L.fv0[kb], i1 = L.fv1[kb], i2 = L.fv2[kb];
+            const v3 pt = normal * depth;
+            const v3 a = lds_w(L, i0), b = lds_w(L, i1), c = lds_w(L, i2);
+            const v3 v0 = b - a, v1 = c - a, v2 = pt - a;
+            const double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
+            const double d20 = dot(v2, v0), d21 = dot(v2, v1);
+            const double den = d00 * d11 - d01 * d01;
+            double bv = 0.0, bw = 0.0;
+            if (den != 0.0) {
+                bv = (d11 * d20 - d01 * d21) / den;
+                bw = (d00 * d21 - d01 * d20) / den;
+            }
+            const double bu = 1.0 - bv - bw;
+            const v3 pa0 = mk(L.ax[i0], L.ay[i0], L.az[i0]), pa1 = mk(L.ax[i1], L.ay[i1], L.az[i1]);
+            const v3 pa2 = mk(L.ax[i2], L.ay[i2], L.az[i2]);
+            const v3 pb0 = mk(L.bx[i0], L.by[i0], L.bz[i0]), pb1 = mk(L.bx[i1], L.by[i1], L.bz[i1]);
+            const v3 pb2 = mk(L.bx[i2], L.by[i2], L.bz[i2]);
+            ca = (pa0 * bu + pa1 * bv) + pa2 * bw;
+            cb = (pb0 * bu + pb1 * bv) + pb2 * bw;
+        }
+        if (lane == 0) {
+            const v3 mtv = normal * depth;
+            o[0] = depth;
+            o[1] = normal.x; o[2] = normal.y; o[3] = normal.z;
+            o[4] = mtv.x; o[5] = mtv.y; o[6] = mtv.z;
+            o[7] = ca.x; o[8] = ca.y; o[9] = ca.z;
+            o[10] = cb.x; o[11] = cb.y; o[12] = cb.z;
+            status_out[pid] = status;
+            if (iters_out) iters_out[pid] = iterations;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace d3d
+
+using namespace d3d;
+
+extern "C" int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
+                             int64_t n, const double *simplex, const int32_t *n_points,
+                             int32_t max_iterations, int32_t max_faces, double epsilon, double *out,
+                             int32_t *status, int32_t *iterations, void *stream) {
+    clear_error();
+    if (int rc = check_scene(scene)) return rc;
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(pair_a && pair_b && simplex && n_points && out && status, "NULL buffer");
+    D3D_CHECK_ARG(max_iterations >= 1 && max_iterations <= kEpaMaxIter,
+                  "max_iterations must be in [1, 64]");
+    D3D_CHECK_ARG(max_faces >= 4 && max_faces <= kEpaMaxFaces, "max_faces must be in [4, 128]");
+    const int64_t grid = n < 65536 ? n : 65536;
+    hipLaunchKernelGGL(epa_kernel, dim3((unsigned)grid), dim3(kEpaBlock), 0, (hipStream_t)stream,
+                       *scene, pair_a, pair_b, n, (const int64_t *)nullptr,
+                       (const int32_t *)nullptr, simplex, n_points, (int)max_iterations,
+                       (int)max_faces, epsilon, out, status, iterations);
+    D3D_CHECK_LAUNCH("epa_kernel");
+    return D3D_OK;
+}

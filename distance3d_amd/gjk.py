@@ -5,8 +5,31 @@ Single-pair calls pack both colliders into a two-collider scene and launch a
 batch of one; batched work goes through :func:`gjk_distance_batch` /
 :func:`gjk_intersection_batch` on a device-resident scene.
 """
+import numpy as np
+
 from . import _native, kernels
 from .scene import SceneBuilder
+
+
+class GjkSimplex(np.ndarray):
+    """The (4, 3) simplex ``gjk`` returns (Minkowski-difference rows, stale
+    rows included as the reference leaves them), carrying what EPA needs to
+    seed from it: ``n_points`` valid rows and the support points on both
+    colliders (``support1``, ``support2``, each (4, 3))."""
+
+    def __new__(cls, Y, P, Q, n_points):
+        obj = np.asarray(Y, dtype=np.float64).reshape(4, 3).copy().view(cls)
+        obj.support1 = np.asarray(P, dtype=np.float64).reshape(4, 3).copy()
+        obj.support2 = np.asarray(Q, dtype=np.float64).reshape(4, 3).copy()
+        obj.n_points = int(n_points)
+        return obj
+
+    def __array_finalize__(self, obj):
+        if obj is None:
+            return
+        self.support1 = getattr(obj, "support1", None)
+        self.support2 = getattr(obj, "support2", None)
+        self.n_points = getattr(obj, "n_points", None)
 
 MAX_FLOAT = _native.MAX_FLOAT
 
@@ -45,7 +68,8 @@ def gjk_distance_jolt(collider1, collider2, tolerance=1e-10, max_distance_square
     dist = float(r.dist.cpu().item())
     cp1 = r.cp_a.cpu().numpy()[0]
     cp2 = r.cp_b.cpu().numpy()[0]
-    simplex = r.simplex.cpu().numpy()[0, 0].copy()
+    S = r.simplex.cpu().numpy()[0]
+    simplex = GjkSimplex(S[0], S[1], S[2], int(r.n_points.cpu().item()))
     return dist, cp1, cp2, simplex
 
 
@@ -76,4 +100,4 @@ def gjk_distance_jolt_iterations(collider1, collider2, tolerance=1e-10,
 
 __all__ = ["gjk", "gjk_distance", "gjk_intersection", "gjk_distance_jolt",
            "gjk_intersection_jolt", "gjk_distance_jolt_iterations", "gjk_distance_batch",
-           "gjk_intersection_batch", "MAX_FLOAT"]
+           "gjk_intersection_batch", "MAX_FLOAT", "GjkSimplex"]

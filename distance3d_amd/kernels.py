@@ -111,3 +111,42 @@ def gjk_intersection_batch(scene, pair_a, pair_b, tolerance=1e-10, max_iteration
         scene.c, ptr(a), ptr(b), n, float(tolerance), int(max_iterations), ptr(hit),
         ptr(status), stream_ptr(stream)), "d3d_gjk_intersection_batch")
     return hit, status
+
+
+class EpaResult:
+    """Outputs of :func:`epa_batch` (device tensors)."""
+
+    def __init__(self, out, status, iterations):
+        self.raw = out
+        self.depth = out[:, 0]
+        self.normal = out[:, 1:4]
+        self.mtv = out[:, 4:7]
+        self.cp_a = out[:, 7:10]
+        self.cp_b = out[:, 10:13]
+        self.status = status
+        self.iterations = iterations
+
+    @property
+    def converged(self):
+        return self.status == 1
+
+
+def epa_batch(scene, pair_a, pair_b, simplex, n_points, max_iterations=64, max_faces=128,
+              epsilon=1e-8, stream=None):
+    """EPA over pairs seeded from GJK simplices ((n,3,4,3) Y/P/Q rows and
+    n_points from :func:`gjk_distance_batch` with ``want_simplex=True``)."""
+    import torch
+    a = _dev_i32(pair_a, scene.device)
+    b = _dev_i32(pair_b, scene.device)
+    n = a.numel()
+    if b.numel() != n:
+        raise ValueError("pair_a and pair_b differ in length")
+    s = _dev_f64(simplex, scene.device).reshape(n, 36)
+    npts = _dev_i32(n_points, scene.device)
+    out = torch.empty((n, 13), dtype=torch.float64, device=scene.device)
+    status = torch.empty(n, dtype=torch.int32, device=scene.device)
+    iters = torch.empty(n, dtype=torch.int32, device=scene.device)
+    check(lib().d3d_epa_batch(scene.c, ptr(a), ptr(b), n, ptr(s), ptr(npts),
+                              int(max_iterations), int(max_faces), float(epsilon), ptr(out),
+                              ptr(status), ptr(iters), stream_ptr(stream)), "d3d_epa_batch")
+    return EpaResult(out, status, iters)

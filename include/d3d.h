@@ -146,6 +146,24 @@ int d3d_gjk_intersection_batch(const d3d_scene *scene, const int32_t *pair_a,
                                int32_t max_iterations, uint8_t *hit, int32_t *status,
                                void *stream);
 
+/*
+ * Batched EPA penetration depth, epa.epa (epa.py:9-78), seeded from the GJK
+ * simplices of d3d_gjk_distance_batch (simplex (n,3,4,3) f64 = Y, P, Q rows,
+ * n_points (n) int32) of intersecting pairs.  One wavefront per pair, polytope
+ * in LDS.
+ *   max_iterations in [1, 64] (epa.py's max_iter), max_faces in [4, 128],
+ *   epsilon: convergence tolerance (1e-8).
+ *   out (n,13) f64: depth, normal[3], mtv[3] = normal * depth, contact point
+ *       on A [3], contact point on B [3] (cp_a - cp_b = mtv when converged).
+ *   status (n) int32: 1 converged, 0 max_iterations reached, 2 face cap,
+ *       3 degenerate seed (flat Minkowski difference), 0x200 bad input.
+ *   iterations (n) int32, optional.
+ */
+int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
+                  int64_t n, const double *simplex, const int32_t *n_points,
+                  int32_t max_iterations, int32_t max_faces, double epsilon, double *out,
+                  int32_t *status, int32_t *iterations, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
