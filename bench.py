@@ -32,7 +32,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--pairs", type=int, default=1 << 20)
     p.add_argument("--seed", type=int, default=1)
-    p.add_argument("--hull-serial-max", type=int, default=16)
+    p.add_argument("--lanes", type=str, default=None,
+                   help="lanes per pair for the 8 work classes, comma separated")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-parity", action="store_true", help="skip the oracle parity check")
     p.add_argument("--cpu-runs", type=int, default=3)
@@ -68,8 +69,11 @@ def main():
         torch.empty(n, dtype=torch.int32, device="cuda"),
         torch.empty(n, dtype=torch.int32, device="cuda"), None, None)
 
+    lanes = [int(x) for x in args.lanes.split(",")] if args.lanes else None
+    ws = torch.empty(kernels.gjk_workspace_bytes(n), dtype=torch.uint8, device="cuda")
+
     def step():
-        kernels.gjk_distance_batch(dev, pa, pb, hull_serial_max=args.hull_serial_max, out=out)
+        kernels.gjk_distance_batch(dev, pa, pb, lanes_per_pair=lanes, out=out, workspace=ws)
 
     for _ in range(args.warmup):
         step()
@@ -120,7 +124,8 @@ def main():
         "config": {"workload": "mixed_pairs (1,048,576 mixed-primitive GJK pairs)",
                    "pairs": n, "seed": args.seed, "hull_vertices": "log-uniform 8..4096",
                    "query": "GJK distance + closest points", "epa": False,
-                   "hull_serial_max": args.hull_serial_max,
+                   "lanes_per_pair": lanes or list(__import__("distance3d_amd._native",
+                                                              fromlist=["x"]).DEFAULT_LANES),
                    "mean_iterations": round(float(iters.mean()), 3)},
         "roofline": roofline,
     }

@@ -51,6 +51,7 @@ class d3d_gjk_params(ctypes.Structure):
         ("sanity_check", ctypes.c_double),
         ("max_iterations", ctypes.c_int32),
         ("hull_serial_max", ctypes.c_int32),
+        ("lanes_per_pair", ctypes.c_int32 * 8),
     ]
 
 
@@ -66,6 +67,8 @@ _SIGNATURES = {
     "d3d_aabb_batch": [_SCENE, _P, _I64, _P, _P],
     "d3d_gjk_distance_batch": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params),
                                _P, _P, _P, _P, _P, _P, _P, _P],
+    "d3d_gjk_distance_batch_ws": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params),
+                                  _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
     "d3d_gjk_intersection_batch": [_SCENE, _P, _P, _I64, _D, _I32, _P, _P, _P],
     "d3d_epa_batch": [_SCENE, _P, _P, _I64, _P, _P, _I32, _I32, _D, _P, _P, _P, _P],
 }
@@ -77,7 +80,8 @@ _initialised = False
 
 def exported_symbols():
     """Names of the C-ABI functions this binding uses."""
-    return ["d3d_abi_version", "d3d_last_error", "d3d_init"] + list(_SIGNATURES)
+    return (["d3d_abi_version", "d3d_last_error", "d3d_init", "d3d_gjk_workspace_bytes"]
+            + list(_SIGNATURES))
 
 
 def load_library(path=LIB_PATH):
@@ -94,6 +98,8 @@ def load_library(path=LIB_PATH):
             lib.d3d_last_error.restype = ctypes.c_char_p
             lib.d3d_init.argtypes = [ctypes.c_int]
             lib.d3d_init.restype = ctypes.c_int
+            lib.d3d_gjk_workspace_bytes.argtypes = [ctypes.c_int64]
+            lib.d3d_gjk_workspace_bytes.restype = ctypes.c_int64
             for name, argtypes in _SIGNATURES.items():
                 fn = getattr(lib, name)
                 fn.argtypes = argtypes
@@ -134,14 +140,22 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+DEFAULT_LANES = (1, 1, 1, 1, 2, 4, 8, 16)
+
+
 def gjk_params(tolerance=1e-10, max_distance_squared=100000.0, sanity_check=1e-8,
-               max_iterations=128, hull_serial_max=16):
+               max_iterations=128, hull_serial_max=16, lanes_per_pair=None):
     p = d3d_gjk_params()
     p.tolerance = float(tolerance)
     p.max_distance_squared = float(max_distance_squared)
     p.sanity_check = float(sanity_check)
     p.max_iterations = int(max_iterations)
     p.hull_serial_max = int(hull_serial_max)
+    lanes = DEFAULT_LANES if lanes_per_pair is None else lanes_per_pair
+    if len(lanes) != 8:
+        raise ValueError("lanes_per_pair needs 8 entries (one per work class)")
+    for c, g in enumerate(lanes):
+        p.lanes_per_pair[c] = int(g)
     return p
 
 

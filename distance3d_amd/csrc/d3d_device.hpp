@@ -35,6 +35,20 @@ __device__ __forceinline__ double dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y
 __device__ __forceinline__ v3 cross(v3 a, v3 b) {
     return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
+// Component-wise select.  A ternary on the aggregate would become a select
+// between pointers plus a copy, which keeps every v3 it touches in scratch
+// memory; scalar selects keep them in VGPRs.
+__device__ __forceinline__ v3 vsel(bool c, v3 a, v3 b) {
+    return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
+}
+__device__ __forceinline__ v3 sel4(int k, v3 a, v3 b, v3 c, v3 d) {
+    v3 r = a;
+    r = vsel(k == 1, b, r);
+    r = vsel(k == 2, c, r);
+    r = vsel(k == 3, d, r);
+    return r;
+}
+
 // scalar_triple_product (utils.py:33-73)
 __device__ __forceinline__ double stp(v3 a, v3 b, v3 c) { return dot(a, cross(b, c)); }
 
@@ -54,6 +68,14 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
     unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Hides a value from the optimiser so loads addressed by it are redone where
+// they appear instead of being hoisted out of a loop (keeps loop-carried
+// register state small; the reloads hit L1/L2).
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
 }
 
 __device__ __forceinline__ int readlane_i32(int v, int lane) {
@@ -299,7 +321,7 @@ __device__ __forceinline__ v3 support_analytic(const Shape &h, v3 d) {
             const double k = r / n;
             dp = mk(dp.x * k, dp.y * k, dp.z * k);
         }
-        const v3 p = (dot(l, dp) >= l.z * ht) ? dp : mk(0.0, 0.0, ht);
+        const v3 p = vsel(dot(l, dp) >= l.z * ht, dp, mk(0.0, 0.0, ht));
         return to_world(h, p);
     }
     case D3D_DISK: {  // geometry.py:349-383
@@ -465,9 +487,9 @@ __device__ __forceinline__ v3 cp_line(v3 a, v3 b, int &set) {
 }
 
 // closest_point_triangle (_gjk_jolt.py:415-523)
-__device__ __noinline__ v3 cp_triangle(v3 a, v3 b, v3 c, int &set) {
+__device__ __forceinline__ v3 cp_triangle(v3 a, v3 b, v3 c, int &set) {
     const v3 ab = b - a, ac = c - a, bc = c - b;
-    const v3 n = (dot(bc, bc) < dot(ac, ac)) ? cross(ab, bc) : cross(ab, ac);
+    const v3 n = vsel(dot(bc, bc) < dot(ac, ac), cross(ab, bc), cross(ab, ac));
     const double nn = dot(n, n);
     if (nn < kEpsSqr) {
         int s, ns;
@@ -533,119 +555,121 @@ __device__ __noinline__ v3 cp_triangle(v3 a, v3 b, v3 c, int &set) {
     return mk(n.x * s / den, n.y * s / den, n.z * s / den);
 }
 
-// closest_point_tetrahedron with origin_outside_of_tetrahedron_planes
-// (_gjk_jolt.py:526-631)
-__device__ __forceinline__ v3 cp_tetra(v3 a, v3 b, v3 c, v3 d, int &set) {
+// origin_outside_of_tetrahedron_planes (_gjk_jolt.py:526-570) as a bit mask
+// over the faces ABC, ACD, ADB, BDC.
+__device__ __forceinline__ int outside_planes_mask(v3 a, v3 b, v3 c, v3 d) {
     const v3 ab = b - a, ac = c - a, ad = d - a, bd = d - b, bc = c - b;
     const v3 abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab), bdc = cross(bd, bc);
     const double sp0 = dot(a, abc), sp1 = dot(a, acd), sp2 = dot(a, adb), sp3 = dot(b, bdc);
     const double sd0 = dot(ad, abc), sd1 = dot(ab, acd), sd2 = dot(ac, adb), sd3 = -dot(ab, bdc);
-    bool o0, o1, o2, o3;
-    if (sd0 > 0.0 && sd1 > 0.0 && sd2 > 0.0 && sd3 > 0.0) {
-        o0 = sp0 >= -kEps;
-        o1 = sp1 >= -kEps;
-        o2 = sp2 >= -kEps;
-        o3 = sp3 >= -kEps;
-    } else if (sd0 < 0.0 && sd1 < 0.0 && sd2 < 0.0 && sd3 < 0.0) {
-        o0 = sp0 <= kEps;
-        o1 = sp1 <= kEps;
-        o2 = sp2 <= kEps;
-        o3 = sp3 <= kEps;
-    } else {
-        o0 = o1 = o2 = o3 = true;
-    }
-    int s = 15, ns;
+    if (sd0 > 0.0 && sd1 > 0.0 && sd2 > 0.0 && sd3 > 0.0)
+        return (sp0 >= -kEps ? 1 : 0) | (sp1 >= -kEps ? 2 : 0) | (sp2 >= -kEps ? 4 : 0) |
+               (sp3 >= -kEps ? 8 : 0);
+    if (sd0 < 0.0 && sd1 < 0.0 && sd2 < 0.0 && sd3 < 0.0)
+        return (sp0 <= kEps ? 1 : 0) | (sp1 <= kEps ? 2 : 0) | (sp2 <= kEps ? 4 : 0) |
+               (sp3 <= kEps ? 8 : 0);
+    return 15;  // mixed signs: degenerate tetrahedron, test every face
+}
+
+// closest_point_tetrahedron (_gjk_jolt.py:573-631) and, for three points,
+// closest_point_triangle: the faces ABC, ACD, ADB, BDC are visited in order by
+// one non-unrolled loop (a single copy of the triangle code keeps register
+// pressure bounded); face 0 always sets the result, later faces only when
+// strictly closer; the triangle's feature bits are remapped to the
+// tetrahedron's vertex order.
+__device__ __forceinline__ v3 cp_faces(v3 y0, v3 y1, v3 y2, v3 y3, int mask, int &set) {
+    int s = 15;
     v3 cp = mk(0.0, 0.0, 0.0);
     double best = kMaxFloat;
-    if (o0) {
-        cp = cp_triangle(a, b, c, s);
-        best = dot(cp, cp);
-    }
-    if (o1) {
-        const v3 q = cp_triangle(a, c, d, ns);
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+        if (!((mask >> f) & 1)) continue;
+        const v3 a = vsel(f == 3, y1, y0);
+        const v3 b = sel4(f, y1, y2, y3, y3);
+        const v3 c = sel4(f, y2, y3, y1, y2);
+        int ns;
+        const v3 q = cp_triangle(a, b, c, ns);
         const double dd = dot(q, q);
-        if (dd < best) {
+        if (f == 0 || dd < best) {
             best = dd;
             cp = q;
-            s = (ns & 1) + ((ns & 6) << 1);
-        }
-    }
-    if (o2) {
-        const v3 q = cp_triangle(a, d, b, ns);
-        const double dd = dot(q, q);
-        if (dd < best) {
-            best = dd;
-            cp = q;
-            s = (ns & 1) + ((ns & 2) << 2) + ((ns & 4) >> 1);
-        }
-    }
-    if (o3) {
-        const v3 q = cp_triangle(b, d, c, ns);
-        const double dd = dot(q, q);
-        if (dd < best) {
-            cp = q;
-            s = ((ns & 1) << 1) + ((ns & 2) << 2) + (ns & 4);
+            s = (f == 0) ? ns
+              : (f == 1) ? (ns & 1) + ((ns & 6) << 1)
+              : (f == 2) ? (ns & 1) + ((ns & 2) << 2) + ((ns & 4) >> 1)
+                         : ((ns & 1) << 1) + ((ns & 2) << 2) + (ns & 4);
         }
     }
     set = s;
     return cp;
 }
 
-// Simplex of up to four points with their supports on A and B.  Rows at
-// index >= n keep stale contents exactly as the reference's in-place
-// compaction leaves them.
+// Minkowski-difference simplex Y of up to four points, held in named members
+// so it stays in VGPRs.  Rows at index >= n keep stale contents exactly as
+// the reference's in-place compaction leaves them.  The support points on A
+// and B that go with each row (P, Q) are kept by the caller (see
+// d3d_gjk_queue.hip), moved with the same compaction map.
 struct Simplex {
-    v3 Y[4], P[4], Q[4];
+    v3 Y0, Y1, Y2, Y3;
     int n;
 };
 
-__device__ __forceinline__ void simplex_append(Simplex &S, v3 y, v3 p, v3 q) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (S.n == k) {
-            S.Y[k] = y;
-            S.P[k] = p;
-            S.Q[k] = q;
-        }
-    }
+__device__ __forceinline__ void simplex_zero(Simplex &S) {
+    S.Y0 = mk(0.0, 0.0, 0.0);
+    S.Y1 = mk(0.0, 0.0, 0.0);
+    S.Y2 = mk(0.0, 0.0, 0.0);
+    S.Y3 = mk(0.0, 0.0, 0.0);
+    S.n = 0;
+}
+
+// Y[n] = y; n += 1
+__device__ __forceinline__ void simplex_append(Simplex &S, v3 y) {
+    S.Y0 = vsel(S.n == 0, y, S.Y0);
+    S.Y1 = vsel(S.n == 1, y, S.Y1);
+    S.Y2 = vsel(S.n == 2, y, S.Y2);
+    S.Y3 = vsel(S.n >= 3, y, S.Y3);
     S.n += 1;
 }
 
-// update_simplex_ypq (_gjk_jolt.py:654-664); with_pq == false is
-// update_simplex_y (:643-651).
-template <bool with_pq>
-__device__ __forceinline__ void simplex_compact(Simplex &S, int set) {
-    Simplex T = S;
-    int m = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const bool keep = (k < S.n) && ((set >> k) & 1);
-#pragma unroll
-        for (int j = 0; j <= k; ++j) {
-            if (keep && m == j) {
-                T.Y[j] = S.Y[k];
-                if (with_pq) {
-                    T.P[j] = S.P[k];
-                    T.Q[j] = S.Q[k];
-                }
-            }
-        }
-        m += keep ? 1 : 0;
+// Order-preserving compaction map of update_simplex_ypq (_gjk_jolt.py:654-664):
+// new row j (< m) takes old row k_j.
+struct CompactMap {
+    int k0, k1, k2, m;
+    bool moved;
+};
+
+// update_simplex_y / update_simplex_ypq on Y (_gjk_jolt.py:643-664); slots >=
+// the new count are left as they were.  Returns the map to apply to P, Q.
+__device__ __forceinline__ CompactMap simplex_compact(Simplex &S, int set) {
+    set &= (1 << S.n) - 1;
+    CompactMap c;
+    c.m = __popc(set);
+    c.k0 = 0;
+    c.k1 = 1;
+    c.k2 = 2;
+    c.moved = set != (1 << c.m) - 1;
+    if (c.moved) {
+        c.k0 = __ffs(set) - 1;
+        const int r1 = set & (set - 1);
+        c.k1 = r1 ? __ffs(r1) - 1 : 3;
+        const int r2 = r1 & (r1 - 1);
+        c.k2 = r2 ? __ffs(r2) - 1 : 3;
+        const v3 y0 = sel4(c.k0, S.Y0, S.Y1, S.Y2, S.Y3);
+        const v3 y1 = sel4(c.k1, S.Y0, S.Y1, S.Y2, S.Y3);
+        const v3 y2 = sel4(c.k2, S.Y0, S.Y1, S.Y2, S.Y3);
+        S.Y0 = y0;
+        S.Y1 = vsel(c.m > 1, y1, S.Y1);
+        S.Y2 = vsel(c.m > 2, y2, S.Y2);
     }
-    T.n = m;
-    S = T;
+    S.n = c.m;
+    return c;
 }
 
 // max_y_length_squared (_gjk_jolt.py:634-640)
 __device__ __forceinline__ double max_y_len_sq(const Simplex &S) {
-    double m = dot(S.Y[0], S.Y[0]);
-#pragma unroll
-    for (int k = 1; k < 4; ++k) {
-        if (k < S.n) {
-            const double l = dot(S.Y[k], S.Y[k]);
-            m = (l > m) ? l : m;
-        }
-    }
+    double m = dot(S.Y0, S.Y0);
+    if (S.n > 1) { const double l = dot(S.Y1, S.Y1); m = (l > m) ? l : m; }
+    if (S.n > 2) { const double l = dot(S.Y2, S.Y2); m = (l > m) ? l : m; }
+    if (S.n > 3) { const double l = dot(S.Y3, S.Y3); m = (l > m) ? l : m; }
     return m;
 }
 
@@ -656,13 +680,12 @@ __device__ __forceinline__ bool closest_to_origin(const Simplex &S, double prev,
     int s;
     if (S.n == 1) {
         s = 1;
-        p = S.Y[0];
+        p = S.Y0;
     } else if (S.n == 2) {
-        p = cp_line(S.Y[0], S.Y[1], s);
-    } else if (S.n == 3) {
-        p = cp_triangle(S.Y[0], S.Y[1], S.Y[2], s);
+        p = cp_line(S.Y0, S.Y1, s);
     } else {
-        p = cp_tetra(S.Y[0], S.Y[1], S.Y[2], S.Y[3], s);
+        const int mask = (S.n == 3) ? 1 : outside_planes_mask(S.Y0, S.Y1, S.Y2, S.Y3);
+        p = cp_faces(S.Y0, S.Y1, S.Y2, S.Y3, mask, s);
     }
     const double l = dot(p, p);
     if (l < prev) {
@@ -672,29 +695,6 @@ __device__ __forceinline__ bool closest_to_origin(const Simplex &S, double prev,
         return true;
     }
     return false;
-}
-
-// calculate_closest_points (_gjk_jolt.py:667-687)
-__device__ __forceinline__ void closest_points(const Simplex &S, v3 &a, v3 &b) {
-    if (S.n == 1) {
-        a = S.P[0];
-        b = S.Q[0];
-    } else if (S.n == 2) {
-        double u, v;
-        bary_line(S.Y[0], S.Y[1], u, v);
-        a = S.P[0] * u + S.P[1] * v;
-        b = S.Q[0] * u + S.Q[1] * v;
-    } else if (S.n == 3) {
-        double u, v, w;
-        bary_plane(S.Y[0], S.Y[1], S.Y[2], u, v, w);
-        a = (S.P[0] * u + S.P[1] * v) + S.P[2] * w;
-        b = (S.Q[0] * u + S.Q[1] * v) + S.Q[2] * w;
-    } else {
-        double u, v, w, x;
-        bary_tetra(S.Y[0], S.Y[1], S.Y[2], S.Y[3], u, v, w, x);
-        a = ((S.P[0] * u + S.P[1] * v) + S.P[2] * w) + S.P[3] * x;
-        b = ((S.Q[0] * u + S.Q[1] * v) + S.Q[2] * w) + S.Q[3] * x;
-    }
 }
 
 }  // namespace d3d

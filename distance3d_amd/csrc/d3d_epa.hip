@@ -85,7 +85,7 @@ __device__ bool epa_seed(const d3d_scene &s, const Shape &SA, const Shape &SB, S
         v3 bw = V.W[0], bpa = V.A[0], bpb = V.B[0];
         for (int k = 0; k < 6; ++k) {
             const double sg = (k & 1) ? -1.0 : 1.0;
-            const v3 d = (k < 2) ? mk(sg, 0.0, 0.0) : ((k < 4) ? mk(0.0, sg, 0.0) : mk(0.0, 0.0, sg));
+            const v3 d = mk(k < 2 ? sg : 0.0, (k >= 2 && k < 4) ? sg : 0.0, k >= 4 ? sg : 0.0);
             const v3 pa = support_uniform(s, SA, d), pb = support_uniform(s, SB, -d);
             const v3 w = pa - pb;
             const v3 e = w - V.W[0];
@@ -107,14 +107,15 @@ __device__ bool epa_seed(const d3d_scene &s, const Shape &SA, const Shape &SB, S
     if (n == 2) {
         const v3 ab = V.W[1] - V.W[0];
         const double ax = fabs(ab.x), ay = fabs(ab.y), az = fabs(ab.z);
-        const v3 e = (ax <= ay && ax <= az) ? mk(1, 0, 0) : ((ay <= az) ? mk(0, 1, 0) : mk(0, 0, 1));
+        const bool ex = ax <= ay && ax <= az, ey = !ex && ay <= az;
+        const v3 e = mk(ex ? 1.0 : 0.0, ey ? 1.0 : 0.0, (!ex && !ey) ? 1.0 : 0.0);
         const v3 p1 = cross(ab, e);
         const v3 p2 = cross(ab, p1);
         int best = -1;
         double bl = 0.0;
         v3 bw = V.W[0], bpa = V.A[0], bpb = V.B[0];
         for (int k = 0; k < 4; ++k) {
-            const v3 c = (k == 0) ? p1 : ((k == 1) ? -p1 : ((k == 2) ? p2 : -p2));
+            const v3 c = sel4(k, p1, -p1, p2, -p2);
             const v3 pa = support_uniform(s, SA, c), pb = support_uniform(s, SB, -c);
             const v3 w = pa - pb;
             const v3 cr = cross(ab, w - V.W[0]);
@@ -199,9 +200,9 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                 t = V.B[1]; V.B[1] = V.B[2]; V.B[2] = t;
             }
             if (lane < 4) {
-                const v3 w = (lane == 0) ? V.W[0] : (lane == 1) ? V.W[1] : (lane == 2) ? V.W[2] : V.W[3];
-                const v3 a = (lane == 0) ? V.A[0] : (lane == 1) ? V.A[1] : (lane == 2) ? V.A[2] : V.A[3];
-                const v3 b = (lane == 0) ? V.B[0] : (lane == 1) ? V.B[1] : (lane == 2) ? V.B[2] : V.B[3];
+                const v3 w = sel4(lane, V.W[0], V.W[1], V.W[2], V.W[3]);
+                const v3 a = sel4(lane, V.A[0], V.A[1], V.A[2], V.A[3]);
+                const v3 b = sel4(lane, V.B[0], V.B[1], V.B[2], V.B[3]);
                 L.wx[lane] = w.x; L.wy[lane] = w.y; L.wz[lane] = w.z;
                 L.ax[lane] = a.x; L.ay[lane] = a.y; L.az[lane] = a.z;
                 L.bx[lane] = b.x; L.by[lane] = b.y; L.bz[lane] = b.z;
@@ -214,9 +215,9 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                 i0 = (lane == 3) ? 1 : 0;
                 i1 = (lane == 0) ? 1 : (lane == 1) ? 3 : (lane == 2) ? 2 : 3;
                 i2 = (lane == 0) ? 2 : (lane == 1) ? 1 : (lane == 2) ? 3 : 2;
-                const v3 w0 = (i0 == 0) ? V.W[0] : V.W[1];
-                const v3 w1 = (i1 == 1) ? V.W[1] : (i1 == 2) ? V.W[2] : V.W[3];
-                const v3 w2 = (i2 == 1) ? V.W[1] : (i2 == 2) ? V.W[2] : V.W[3];
+                const v3 w0 = sel4(i0, V.W[0], V.W[1], V.W[2], V.W[3]);
+                const v3 w1 = sel4(i1, V.W[0], V.W[1], V.W[2], V.W[3]);
+                const v3 w2 = sel4(i2, V.W[0], V.W[1], V.W[2], V.W[3]);
                 v3 nn;
                 double dd;
                 fok = face_normal(w0, w1, w2, nn, dd);

@@ -1,6 +1,8 @@
 // d3d_gjk.hip -- batched GJK distance / intersection, support points and
 // AABBs for gfx950.
 //
+// (The batched GJK distance pipeline is in d3d_gjk_queue.hip.)
+//
 // Kernel shape: one lane per pair, 64 pairs per wavefront.  The FP64 simplex
 // (Y, P, Q of up to four points) lives in VGPRs of its lane; support points of
 // analytic colliders and of small hulls are computed by the lane itself, while
@@ -15,136 +17,6 @@ namespace d3d {
 
 constexpr int kBlock = 256;
 constexpr int kStatusBadInput = 0x200;
-
-// gjk_distance_jolt (_gjk_jolt.py:138-221) and _distance_loop (:224-288).
-__global__ __launch_bounds__(kBlock) void gjk_distance_kernel(
-    d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, int64_t n,
-    d3d_gjk_params prm, double *__restrict__ dist, double *__restrict__ cpa,
-    double *__restrict__ cpb, int32_t *__restrict__ status, int32_t *__restrict__ iters,
-    double *__restrict__ simplex_out, int32_t *__restrict__ npts_out) {
-    const int64_t pid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = pid < n;
-    Shape A = empty_shape(), B = empty_shape();
-    bool ok = false;
-    if (valid) {
-        const bool okA = load_checked(s, pa[pid], A);
-        const bool okB = load_checked(s, pb[pid], B);
-        ok = okA && okB;
-    }
-    Simplex S;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) S.Y[k] = S.P[k] = S.Q[k] = mk(0.0, 0.0, 0.0);
-    S.n = 0;
-    v3 d = mk(1.0, 0.0, 0.0);
-    double vv = dot(d, d);
-    double prev = kMaxFloat;
-    const double tol2 = prm.tolerance * prm.tolerance;
-    const int serial_max = prm.hull_serial_max;
-    int state = -1, it = 0;
-    bool active = ok;
-
-    while (__ballot(active) != 0ull) {
-        if (active && it >= prm.max_iterations) {
-            state = D3D_ITER_CAP;
-            active = false;
-        }
-        const v3 p = support_wave(s, A, d, active, serial_max);
-        const v3 q = support_wave(s, B, -d, active, serial_max);
-        if (!active) continue;
-        ++it;
-        const v3 w = p - q;
-        const double dw = dot(d, w);
-        if (dw < 0.0 && dw * dw > vv * prm.max_distance_squared) {
-            state = D3D_CLIPPED;
-            active = false;
-            continue;
-        }
-        simplex_append(S, w, p, q);
-        v3 nvec;
-        double nvv;
-        int set;
-        if (closest_to_origin(S, prev, nvec, nvv, set)) {
-            d = nvec;
-            vv = nvv;
-        } else {
-            S.n -= 1;
-            set = (1 << S.n) - 1;
-        }
-        if (set == 15) {
-            vv = 0.0;
-            state = D3D_INTERSECT;
-            active = false;
-            continue;
-        }
-        simplex_compact<true>(S, set);
-        if (vv <= tol2 || vv <= kEps * max_y_len_sq(S)) {
-            vv = 0.0;
-            state = D3D_INTERSECT;
-            active = false;
-            continue;
-        }
-        d = -d;
-        if (prev - vv <= kEps * prev) {
-            state = D3D_SEPARATED;
-            active = false;
-            continue;
-        }
-        prev = vv;
-    }
-
-    if (!valid) return;
-    if (!ok) {
-        dist[pid] = NAN;
-        cpa[3 * pid] = cpa[3 * pid + 1] = cpa[3 * pid + 2] = NAN;
-        cpb[3 * pid] = cpb[3 * pid + 1] = cpb[3 * pid + 2] = NAN;
-        status[pid] = kStatusBadInput;
-        if (iters) iters[pid] = 0;
-        if (npts_out) npts_out[pid] = 0;
-        return;
-    }
-    if (iters) iters[pid] = it;
-    if (npts_out) npts_out[pid] = S.n;
-    if (simplex_out) {
-        double *o = simplex_out + 36 * pid;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            o[3 * k] = S.Y[k].x;
-            o[3 * k + 1] = S.Y[k].y;
-            o[3 * k + 2] = S.Y[k].z;
-            o[12 + 3 * k] = S.P[k].x;
-            o[13 + 3 * k] = S.P[k].y;
-            o[14 + 3 * k] = S.P[k].z;
-            o[24 + 3 * k] = S.Q[k].x;
-            o[25 + 3 * k] = S.Q[k].y;
-            o[26 + 3 * k] = S.Q[k].z;
-        }
-    }
-    if (state == D3D_CLIPPED) {
-        dist[pid] = kMaxFloat;
-        cpa[3 * pid] = cpa[3 * pid + 1] = cpa[3 * pid + 2] = NAN;
-        cpb[3 * pid] = cpb[3 * pid + 1] = cpb[3 * pid + 2] = NAN;
-        status[pid] = D3D_CLIPPED;
-        return;
-    }
-    v3 a, b;
-    closest_points(S, a, b);
-    int st = state;
-    if (!(fabs(dot(d, d) - vv) < prm.sanity_check)) st |= D3D_SANITY_FAILED;
-    const double dd = sqrt(vv);
-    if (dd < kEps) {
-        const v3 m = (a + b) * 0.5;
-        a = m;
-        b = m;
-    }
-    dist[pid] = dd;
-    cpa[3 * pid] = a.x;
-    cpa[3 * pid + 1] = a.y;
-    cpa[3 * pid + 2] = a.z;
-    cpb[3 * pid] = b.x;
-    cpb[3 * pid + 1] = b.y;
-    cpb[3 * pid + 2] = b.z;
-    status[pid] = st;
-}
 
 // gjk_intersection_jolt (_gjk_jolt.py:29-80) and _intersection_loop (:83-135).
 __global__ __launch_bounds__(kBlock) void gjk_intersection_kernel(
@@ -161,9 +33,7 @@ __global__ __launch_bounds__(kBlock) void gjk_intersection_kernel(
         ok = okA && okB;
     }
     Simplex S;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) S.Y[k] = S.P[k] = S.Q[k] = mk(0.0, 0.0, 0.0);
-    S.n = 0;
+    simplex_zero(S);
     v3 d = mk(1.0, 0.0, 0.0);
     double prev = kMaxFloat;
     const double tol2 = tolerance * tolerance;
@@ -185,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void gjk_intersection_kernel(
             active = false;
             continue;
         }
-        simplex_append(S, w, w, w);
+        simplex_append(S, w);
         v3 v;
         double vv;
         int set;
@@ -206,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void gjk_intersection_kernel(
             continue;
         }
         prev = vv;
-        simplex_compact<false>(S, set);
+        simplex_compact(S, set);
     }
     if (!valid) return;
     if (!ok) {
@@ -396,6 +266,7 @@ static d3d_gjk_params default_params() {
     p.sanity_check = 1e-8;
     p.max_iterations = 128;
     p.hull_serial_max = 16;
+    for (int c = 0; c < 8; ++c) p.lanes_per_pair[c] = 0;
     return p;
 }
 
@@ -423,26 +294,6 @@ extern "C" int d3d_aabb_batch(const d3d_scene *scene, const int32_t *shape, int6
     hipLaunchKernelGGL(aabb_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
                        *scene, shape, n, out);
     D3D_CHECK_LAUNCH("aabb_kernel");
-    return D3D_OK;
-}
-
-extern "C" int d3d_gjk_distance_batch(const d3d_scene *scene, const int32_t *pair_a,
-                                      const int32_t *pair_b, int64_t n,
-                                      const d3d_gjk_params *params, double *dist, double *cp_a,
-                                      double *cp_b, int32_t *status, int32_t *iterations,
-                                      double *simplex, int32_t *n_points, void *stream) {
-    clear_error();
-    if (int rc = check_scene(scene)) return rc;
-    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
-    if (n == 0) return D3D_OK;
-    D3D_CHECK_ARG(pair_a && pair_b && dist && cp_a && cp_b && status, "NULL buffer");
-    const d3d_gjk_params prm = params ? *params : default_params();
-    D3D_CHECK_ARG(prm.max_iterations > 0, "max_iterations must be > 0");
-    D3D_CHECK_ARG(prm.hull_serial_max >= 0, "hull_serial_max must be >= 0");
-    hipLaunchKernelGGL(gjk_distance_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0,
-                       (hipStream_t)stream, *scene, pair_a, pair_b, n, prm, dist, cp_a, cp_b, status,
-                       iterations, simplex, n_points);
-    D3D_CHECK_LAUNCH("gjk_distance_kernel");
     return D3D_OK;
 }
 

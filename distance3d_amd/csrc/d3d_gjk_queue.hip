@@ -1,0 +1,680 @@
+// d3d_gjk_queue.hip -- batched GJK distance as a sorted, queue-fed pipeline.
+//
+// 1. classify + counting sort (device): every pair gets a key
+//        work class (total hull vertices of the pair, 8 classes)
+//        x collider type of A x collider type of B
+//    and the pair indices are scattered into `order`, grouped by key.  Pairs
+//    of one class have similar support-sweep cost and, within a class, run
+//    next to pairs of the same type combination, so the type switch of the
+//    analytic supports stays wave-uniform.
+// 2. one persistent kernel per class, G lanes per pair (G from the tuning
+//    table; 1 for analytic and small-hull pairs, more for big hulls).  The G
+//    lanes of a pair hold identical FP64 simplex state; hull sweeps stride
+//    over the vertices by G and reduce inside the group.  Each group takes
+//    its next pair from the class's device cursor as soon as its current
+//    pair terminates, so lanes do not idle on the slowest pair of a wave.
+// Counts and offsets stay on the device: no host synchronisation inside a
+// batch.
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "d3d_device.hpp"
+#include "d3d_runtime.hpp"
+
+namespace d3d {
+
+constexpr int kQBlock = 256;
+constexpr int kNumClasses = 8;
+constexpr int kNumTypes = 9;
+constexpr int kNumKeys = kNumClasses * kNumTypes * kNumTypes;
+constexpr int kBadInput = 0x200;
+
+// work class from the total hull vertex count of a pair
+__device__ __forceinline__ int work_class(int nverts) {
+    if (nverts == 0) return 0;
+    if (nverts <= 32) return 1;
+    if (nverts <= 128) return 2;
+    if (nverts <= 512) return 3;
+    if (nverts <= 1024) return 4;
+    if (nverts <= 2048) return 5;
+    if (nverts <= 4096) return 6;
+    return 7;
+}
+
+__device__ __forceinline__ int pair_key(const d3d_scene &s, int a, int b) {
+    if (a < 0 || b < 0 || (int64_t)a >= s.n_shapes || (int64_t)b >= s.n_shapes) return 0;
+    const int4 A = *reinterpret_cast<const int4 *>(s.shape_i32 + 4 * (int64_t)a);
+    const int4 B = *reinterpret_cast<const int4 *>(s.shape_i32 + 4 * (int64_t)b);
+    const int ta = (A.x >= 0 && A.x < kNumTypes) ? A.x : 0;
+    const int tb = (B.x >= 0 && B.x < kNumTypes) ? B.x : 0;
+    const int na = (ta == D3D_HULL) ? (A.y > 0 ? A.y : 0) : 0;
+    const int nb = (tb == D3D_HULL) ? (B.y > 0 ? B.y : 0) : 0;
+    const int cls = work_class(na + nb);
+    return (cls * kNumTypes + ta) * kNumTypes + tb;
+}
+
+// Workspace layout (int32): hist[K] key_off[K] key_cur[K] cls_begin[C]
+// cls_count[C] cls_cursor[C] pad order[n]
+struct QueueWs {
+    int32_t *hist, *key_off, *key_cur, *cls_begin, *cls_count, *cls_cursor, *order;
+};
+
+__host__ __device__ inline int64_t ws_header_ints() {
+    return ((3 * kNumKeys + 3 * kNumClasses) + 63) / 64 * 64;
+}
+
+inline QueueWs ws_view(void *base, int64_t n) {
+    (void)n;
+    int32_t *p = (int32_t *)base;
+    QueueWs w;
+    w.hist = p;
+    w.key_off = p + kNumKeys;
+    w.key_cur = p + 2 * kNumKeys;
+    w.cls_begin = p + 3 * kNumKeys;
+    w.cls_count = w.cls_begin + kNumClasses;
+    w.cls_cursor = w.cls_count + kNumClasses;
+    w.order = p + ws_header_ints();
+    return w;
+}
+
+// Clears the header with a fill kernel (no memset node, so the sequence can
+// be captured in a graph and replayed).
+__global__ void queue_clear_kernel(int32_t *hdr, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) hdr[i] = 0;
+}
+
+__global__ __launch_bounds__(kQBlock) void queue_hist_kernel(d3d_scene s,
+                                                             const int32_t *__restrict__ pa,
+                                                             const int32_t *__restrict__ pb,
+                                                             int64_t n, int32_t *__restrict__ hist) {
+    __shared__ int32_t h[kNumKeys];
+    for (int i = threadIdx.x; i < kNumKeys; i += kQBlock) h[i] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * kQBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kQBlock)
+        atomicAdd(&h[pair_key(s, pa[i], pb[i])], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kNumKeys; i += kQBlock)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Single-block exclusive scan over keys; class ranges follow from the key
+// order (class-major).
+__global__ __launch_bounds__(kQBlock) void queue_scan_kernel(int32_t *__restrict__ hist,
+                                                             int32_t *__restrict__ key_off,
+                                                             int32_t *__restrict__ cls_begin,
+                                                             int32_t *__restrict__ cls_count) {
+    __shared__ int32_t part[kQBlock];
+    constexpr int per = (kNumKeys + kQBlock - 1) / kQBlock;
+    const int t = threadIdx.x;
+    int local = 0;
+    for (int j = 0; j < per; ++j) {
+        const int k = t * per + j;
+        if (k < kNumKeys) local += hist[k];
+    }
+    part[t] = local;
+    __syncthreads();
+    for (int off = 1; off < kQBlock; off <<= 1) {
+        const int v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - local;
+    for (int j = 0; j < per; ++j) {
+        const int k = t * per + j;
+        if (k < kNumKeys) {
+            key_off[k] = run;
+            run += hist[k];
+        }
+    }
+    __syncthreads();
+    if (t < kNumClasses) {
+        const int k0 = t * kNumTypes * kNumTypes;
+        const int k1 = k0 + kNumTypes * kNumTypes;
+        int c = 0;
+        for (int k = k0; k < k1; ++k) c += hist[k];
+        cls_begin[t] = key_off[k0];
+        cls_count[t] = c;
+    }
+}
+
+__global__ __launch_bounds__(kQBlock) void queue_scatter_kernel(
+    d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, int64_t n,
+    const int32_t *__restrict__ key_off, int32_t *__restrict__ key_cur,
+    int32_t *__restrict__ order) {
+    __shared__ int32_t h[kNumKeys];
+    __shared__ int32_t base[kNumKeys];
+    for (int64_t b0 = (int64_t)blockIdx.x * kQBlock; b0 < n; b0 += (int64_t)gridDim.x * kQBlock) {
+        for (int i = threadIdx.x; i < kNumKeys; i += kQBlock) h[i] = 0;
+        __syncthreads();
+        const int64_t i = b0 + threadIdx.x;
+        int key = 0, rank = 0;
+        if (i < n) {
+            key = pair_key(s, pa[i], pb[i]);
+            rank = atomicAdd(&h[key], 1);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < kNumKeys; k += kQBlock)
+            if (h[k]) base[k] = atomicAdd(&key_cur[k], h[k]);
+        __syncthreads();
+        if (i < n) order[key_off[key] + base[key] + rank] = (int32_t)i;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- GJK state
+
+// Support points on A (P) and B (Q) of the simplex rows, in LDS: they are
+// touched only on append, compaction and at the end, so keeping them out of
+// VGPRs leaves the registers to the simplex solve.  Structure of arrays over
+// the block's threads: component c (slot * 6 + {Px,Py,Pz,Qx,Qy,Qz}) of thread
+// t at base[c * kQBlock + t], conflict-free.
+struct PQ {
+    double *base;
+    __device__ __forceinline__ void put(int slot, v3 p, v3 q) const {
+        double *b = base + slot * 6 * kQBlock;
+        b[0] = p.x;
+        b[kQBlock] = p.y;
+        b[2 * kQBlock] = p.z;
+        b[3 * kQBlock] = q.x;
+        b[4 * kQBlock] = q.y;
+        b[5 * kQBlock] = q.z;
+    }
+    __device__ __forceinline__ v3 P(int slot) const {
+        const double *b = base + slot * 6 * kQBlock;
+        return mk(b[0], b[kQBlock], b[2 * kQBlock]);
+    }
+    __device__ __forceinline__ v3 Q(int slot) const {
+        const double *b = base + slot * 6 * kQBlock;
+        return mk(b[3 * kQBlock], b[4 * kQBlock], b[5 * kQBlock]);
+    }
+    __device__ __forceinline__ void zero() const {
+#pragma unroll
+        for (int c = 0; c < 24; ++c) base[c * kQBlock] = 0.0;
+    }
+    __device__ __forceinline__ void move(int from, int to) const {
+        if (from == to) return;
+        const double *a = base + from * 6 * kQBlock;
+        double *b = base + to * 6 * kQBlock;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) b[c * kQBlock] = a[c * kQBlock];
+    }
+    // applies a compaction map (sources are >= destinations, so moving
+    // rows in increasing order reads every row before it is overwritten)
+    __device__ __forceinline__ void compact(const CompactMap &c) const {
+        if (!c.moved) return;
+        move(c.k0, 0);
+        if (c.m > 1) move(c.k1, 1);
+        if (c.m > 2) move(c.k2, 2);
+    }
+};
+
+struct GjkState {
+    Simplex S;
+    v3 d;
+    double vv, prev;
+    int it, state;
+};
+
+__device__ __forceinline__ void gjk_init(GjkState &g, const PQ &pq) {
+    simplex_zero(g.S);
+    pq.zero();
+    g.d = mk(1.0, 0.0, 0.0);
+    g.vv = dot(g.d, g.d);
+    g.prev = kMaxFloat;
+    g.it = 0;
+    g.state = -1;
+}
+
+// One pass of _distance_loop (_gjk_jolt.py:224-288) given the supports p, q.
+// Returns false once the pair has terminated (g.state set).
+__device__ __forceinline__ bool gjk_distance_step(GjkState &g, const PQ &pq, v3 p, v3 q,
+                                                  double maxd2, double tol2) {
+    ++g.it;
+    const v3 w = p - q;
+    const double dw = dot(g.d, w);
+    if (dw < 0.0 && dw * dw > g.vv * maxd2) {
+        g.state = D3D_CLIPPED;
+        return false;
+    }
+    pq.put(g.S.n < 3 ? g.S.n : 3, p, q);
+    simplex_append(g.S, w);
+    v3 nvec;
+    double nvv;
+    int set;
+    if (closest_to_origin(g.S, g.prev, nvec, nvv, set)) {
+        g.d = nvec;
+        g.vv = nvv;
+    } else {
+        g.S.n -= 1;
+        set = (1 << g.S.n) - 1;
+    }
+    if (set == 15) {
+        g.vv = 0.0;
+        g.state = D3D_INTERSECT;
+        return false;
+    }
+    pq.compact(simplex_compact(g.S, set));
+    if (g.vv <= tol2 || g.vv <= kEps * max_y_len_sq(g.S)) {
+        g.vv = 0.0;
+        g.state = D3D_INTERSECT;
+        return false;
+    }
+    g.d = -g.d;
+    if (g.prev - g.vv <= kEps * g.prev) {
+        g.state = D3D_SEPARATED;
+        return false;
+    }
+    g.prev = g.vv;
+    return true;
+}
+
+struct GjkOut {
+    double *dist, *cpa, *cpb;
+    int32_t *status, *iters;
+    double *simplex;
+    int32_t *npts;
+};
+
+// calculate_closest_points (_gjk_jolt.py:667-687)
+__device__ __forceinline__ void closest_points(const Simplex &S, const PQ &pq, v3 &a, v3 &b) {
+    if (S.n == 1) {
+        a = pq.P(0);
+        b = pq.Q(0);
+    } else if (S.n == 2) {
+        double u, v;
+        bary_line(S.Y0, S.Y1, u, v);
+        a = pq.P(0) * u + pq.P(1) * v;
+        b = pq.Q(0) * u + pq.Q(1) * v;
+    } else if (S.n == 3) {
+        double u, v, w;
+        bary_plane(S.Y0, S.Y1, S.Y2, u, v, w);
+        a = (pq.P(0) * u + pq.P(1) * v) + pq.P(2) * w;
+        b = (pq.Q(0) * u + pq.Q(1) * v) + pq.Q(2) * w;
+    } else {
+        double u, v, w, x;
+        bary_tetra(S.Y0, S.Y1, S.Y2, S.Y3, u, v, w, x);
+        a = ((pq.P(0) * u + pq.P(1) * v) + pq.P(2) * w) + pq.P(3) * x;
+        b = ((pq.Q(0) * u + pq.Q(1) * v) + pq.Q(2) * w) + pq.Q(3) * x;
+    }
+}
+
+// End of gjk_distance_jolt (_gjk_jolt.py:207-221).
+__device__ __forceinline__ void gjk_write(const GjkState &g, const PQ &pq, int64_t pid,
+                                          double sanity, const GjkOut &o) {
+    if (o.iters) o.iters[pid] = g.it;
+    if (o.npts) o.npts[pid] = g.S.n;
+    if (o.simplex) {
+        double *w = o.simplex + 36 * pid;
+        const v3 y[4] = {g.S.Y0, g.S.Y1, g.S.Y2, g.S.Y3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const v3 P = pq.P(k), Q = pq.Q(k);
+            w[3 * k] = y[k].x;
+            w[3 * k + 1] = y[k].y;
+            w[3 * k + 2] = y[k].z;
+            w[12 + 3 * k] = P.x;
+            w[13 + 3 * k] = P.y;
+            w[14 + 3 * k] = P.z;
+            w[24 + 3 * k] = Q.x;
+            w[25 + 3 * k] = Q.y;
+            w[26 + 3 * k] = Q.z;
+        }
+    }
+    if (g.state == D3D_CLIPPED) {
+        o.dist[pid] = kMaxFloat;
+        o.cpa[3 * pid] = o.cpa[3 * pid + 1] = o.cpa[3 * pid + 2] = NAN;
+        o.cpb[3 * pid] = o.cpb[3 * pid + 1] = o.cpb[3 * pid + 2] = NAN;
+        o.status[pid] = D3D_CLIPPED;
+        return;
+    }
+    v3 a, b;
+    closest_points(g.S, pq, a, b);
+    int st = g.state;
+    if (!(fabs(dot(g.d, g.d) - g.vv) < sanity)) st |= D3D_SANITY_FAILED;
+    const double dd = sqrt(g.vv);
+    if (dd < kEps) {
+        const v3 m = (a + b) * 0.5;
+        a = m;
+        b = m;
+    }
+    o.dist[pid] = dd;
+    o.cpa[3 * pid] = a.x;
+    o.cpa[3 * pid + 1] = a.y;
+    o.cpa[3 * pid + 2] = a.z;
+    o.cpb[3 * pid] = b.x;
+    o.cpb[3 * pid + 1] = b.y;
+    o.cpb[3 * pid + 2] = b.z;
+    o.status[pid] = st;
+}
+
+__device__ __forceinline__ void gjk_write_bad(int64_t pid, const GjkOut &o) {
+    o.dist[pid] = NAN;
+    o.cpa[3 * pid] = o.cpa[3 * pid + 1] = o.cpa[3 * pid + 2] = NAN;
+    o.cpb[3 * pid] = o.cpb[3 * pid + 1] = o.cpb[3 * pid + 2] = NAN;
+    o.status[pid] = kBadInput;
+    if (o.iters) o.iters[pid] = 0;
+    if (o.npts) o.npts[pid] = 0;
+}
+
+// ------------------------------------------------------- group supports
+
+// Argmax over one hull by the G lanes of a group (G a power of two <= 64):
+// lane `sub` scans vertices sub, sub + G, ...; the FP32 scores carry the
+// exactness certificate of coop_hull_argmax, with an FP64 rescan by the
+// group when it fails.
+template <int G>
+__device__ __forceinline__ int group_hull_argmax(const d3d_scene &s, int voff, int voff64, int nv,
+                                                 double L, v3 l, int sub) {
+    const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
+    const float *__restrict__ px = s.vx + voff;
+    const float *__restrict__ py = s.vy + voff;
+    const float *__restrict__ pz = s.vz + voff;
+    float m1 = -INFINITY, m2 = -INFINITY;
+    int i1 = 0x7fffffff;
+    int k = sub;
+    for (; k + G < nv; k += 2 * G) {
+        const float x0 = px[k], y0 = py[k], z0 = pz[k];
+        const float x1 = px[k + G], y1 = py[k + G], z1 = pz[k + G];
+        const float s0 = __builtin_fmaf(z0, dz, __builtin_fmaf(y0, dy, x0 * dx));
+        const float s1 = __builtin_fmaf(z1, dz, __builtin_fmaf(y1, dy, x1 * dx));
+        m2 = fmaxf(m2, fminf(s0, m1));
+        if (s0 > m1) { m1 = s0; i1 = k; }
+        m2 = fmaxf(m2, fminf(s1, m1));
+        if (s1 > m1) { m1 = s1; i1 = k + G; }
+    }
+    if (k < nv) {
+        const float s0 = __builtin_fmaf(pz[k], dz, __builtin_fmaf(py[k], dy, px[k] * dx));
+        m2 = fmaxf(m2, fminf(s0, m1));
+        if (s0 > m1) { m1 = s0; i1 = k; }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        const float om1 = __shfl_xor(m1, off);
+        const float om2 = __shfl_xor(m2, off);
+        const int oi = __shfl_xor(i1, off);
+        m2 = fmaxf(fmaxf(m2, om2), fminf(m1, om1));
+        const bool take = (om1 > m1) || (om1 == m1 && oi < i1);
+        m1 = take ? om1 : m1;
+        i1 = take ? oi : i1;
+    }
+    const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
+    const double E = 8.0 * 5.9604644775390625e-08 * L * linf;
+    if ((double)m1 - (double)m2 > 2.0 * E) return i1;
+
+    double best = -INFINITY;
+    int bi = 0x7fffffff;
+    for (int j = sub; j < nv; j += G) {
+        const v3 v = hull_vertex(s, voff, voff64, j);
+        const double val = v.x * l.x + v.y * l.y + v.z * l.z;
+        if (val > best) {
+            best = val;
+            bi = j;
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        const bool take = (ob > best) || (ob == best && oi < bi);
+        best = take ? ob : best;
+        bi = take ? oi : bi;
+    }
+    return bi == 0x7fffffff ? 0 : bi;
+}
+
+template <int G, bool kHull>
+__device__ __forceinline__ v3 support_group(const d3d_scene &s, const Shape &h, v3 d, int sub) {
+    v3 out;
+    if (kHull && h.type == D3D_HULL) {
+        const v3 l = to_local(h, d);
+        const int idx = group_hull_argmax<G>(s, h.voff, h.voff64, h.nv, h.p[0], l, sub);
+        out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));
+    } else {
+        out = support_analytic(h, d);
+    }
+    return apply_margin(h, out, d);
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
+    return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+}
+
+template <int G, bool kHull>
+__global__ __launch_bounds__(kQBlock) void gjk_queue_kernel(
+    d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ cls_begin,
+    const int32_t *__restrict__ cls_count, int32_t *__restrict__ cls_cursor, int cls,
+    d3d_gjk_params prm, GjkOut o) {
+    const int begin = cls_begin[cls];
+    const int count = cls_count[cls];
+    const int lane = lane_id();
+    const int sub = lane & (G - 1);
+    const int leader = lane - sub;
+    const double tol2 = prm.tolerance * prm.tolerance;
+    __shared__ double pq_lds[24 * kQBlock];
+    const PQ pq{pq_lds + threadIdx.x};
+    GjkState g;
+    gjk_init(g, pq);
+    int ia = 0, ib = 0;
+    int64_t pid = 0;
+    bool have = false, exhausted = false;
+    while (true) {
+        const bool need = !have && !exhausted;
+        const unsigned long long m = __ballot(need && sub == 0);
+        if (m != 0ull) {
+            const int first = __ffsll((long long)m) - 1;
+            int base = 0;
+            if (lane == first) base = atomicAdd(&cls_cursor[cls], __popcll(m));
+            base = readlane_i32(base, first);
+            const int rank = __shfl(__popcll(m & lanemask_lt(lane)), leader);
+            if (need) {
+                const int idx = base + rank;
+                if (idx < count) {
+                    pid = order[begin + idx];
+                    ia = pa[pid];
+                    ib = pb[pid];
+                    Shape A, B;
+                    const bool okA = load_checked(s, ia, A);
+                    const bool okB = load_checked(s, ib, B);
+                    const bool ok =
+                        okA && okB && (kHull || (A.type != D3D_HULL && B.type != D3D_HULL));
+                    gjk_init(g, pq);
+                    if (ok) {
+                        have = true;
+                    } else if (sub == 0) {
+                        gjk_write_bad(pid, o);
+                    }
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(have) == 0ull) {
+            if (__ballot(!exhausted) == 0ull) break;
+            continue;
+        }
+        if (have) {
+            bool running = true;
+            if (g.it >= prm.max_iterations) {
+                g.state = D3D_ITER_CAP;
+                running = false;
+            } else {
+                // the records were validated when the pair was taken
+                const Shape A = load_shape(s, opaque(ia));
+                const v3 p = support_group<G, kHull>(s, A, g.d, sub);
+                const Shape B = load_shape(s, opaque(ib));
+                const v3 q = support_group<G, kHull>(s, B, -g.d, sub);
+                running = gjk_distance_step(g, pq, p, q, prm.max_distance_squared, tol2);
+            }
+            if (!running) {
+                if (sub == 0) gjk_write(g, pq, pid, prm.sanity_check, o);
+                have = false;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host
+
+static const int kDefaultLanes[kNumClasses] = {1, 1, 1, 1, 2, 4, 8, 16};
+
+struct WsCache {
+    std::mutex mu;
+    std::map<std::pair<int, void *>, std::pair<void *, size_t>> bufs;
+};
+static WsCache g_ws;
+
+static int workspace_for(void *stream, int64_t n, void **out) {
+    int dev = 0;
+    D3D_CHECK_HIP(hipGetDevice(&dev));
+    const size_t need = (size_t)(ws_header_ints() + n) * sizeof(int32_t);
+    std::lock_guard<std::mutex> lk(g_ws.mu);
+    auto &e = g_ws.bufs[std::make_pair(dev, stream)];
+    if (e.second < need) {
+        if (e.first) {
+            D3D_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+            D3D_CHECK_HIP(hipFree(e.first));
+            e.first = nullptr;
+            e.second = 0;
+        }
+        const size_t grow = need + need / 4;
+        D3D_CHECK_HIP(hipMalloc(&e.first, grow));
+        e.second = grow;
+    }
+    *out = e.first;
+    return D3D_OK;
+}
+
+static int cu_count() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+template <int G, bool kHull>
+static void launch_class(int cls, int64_t n, const d3d_scene &s, const int32_t *pa,
+                         const int32_t *pb, const QueueWs &w, const d3d_gjk_params &prm,
+                         const GjkOut &o, hipStream_t stream) {
+    const int64_t per_block = kQBlock / G;
+    int64_t grid = (n + per_block - 1) / per_block;
+    const int64_t cap = (int64_t)cu_count() * 4;
+    if (grid > cap) grid = cap;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL((gjk_queue_kernel<G, kHull>), dim3((unsigned)grid), dim3(kQBlock), 0, stream,
+                       s, pa, pb, w.order, w.cls_begin, w.cls_count, w.cls_cursor, cls, prm, o);
+}
+
+static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n,
+                      const d3d_gjk_params &prm, const GjkOut &o, void *ws, hipStream_t stream) {
+    const QueueWs w = ws_view(ws, n);
+    const int hdr = (int)ws_header_ints();
+    hipLaunchKernelGGL(queue_clear_kernel, dim3(4), dim3(kQBlock), 0, stream, (int32_t *)ws, hdr);
+    D3D_CHECK_LAUNCH("queue_clear_kernel");
+    int64_t g = (n + kQBlock - 1) / kQBlock;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(queue_hist_kernel, dim3((unsigned)g), dim3(kQBlock), 0, stream, s, pa, pb, n,
+                       w.hist);
+    D3D_CHECK_LAUNCH("queue_hist_kernel");
+    hipLaunchKernelGGL(queue_scan_kernel, dim3(1), dim3(kQBlock), 0, stream, w.hist, w.key_off,
+                       w.cls_begin, w.cls_count);
+    D3D_CHECK_LAUNCH("queue_scan_kernel");
+    hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)g), dim3(kQBlock), 0, stream, s, pa, pb,
+                       n, w.key_off, w.key_cur, w.order);
+    D3D_CHECK_LAUNCH("queue_scatter_kernel");
+    // big-hull classes first so their long-running groups start early
+    for (int cls = kNumClasses - 1; cls >= 0; --cls) {
+        int G = prm.lanes_per_pair[cls] > 0 ? prm.lanes_per_pair[cls] : kDefaultLanes[cls];
+        if (cls == 0) {
+            launch_class<1, false>(cls, n, s, pa, pb, w, prm, o, stream);
+        } else {
+            switch (G) {
+            case 1: launch_class<1, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
+            case 2: launch_class<2, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
+            case 4: launch_class<4, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
+            case 8: launch_class<8, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
+            case 16: launch_class<16, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
+            case 32: launch_class<32, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
+            default: launch_class<64, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
+            }
+        }
+        D3D_CHECK_LAUNCH("gjk_queue_kernel");
+    }
+    return D3D_OK;
+}
+
+}  // namespace d3d
+
+using namespace d3d;
+
+extern "C" int64_t d3d_gjk_workspace_bytes(int64_t n) {
+    if (n < 0) return -1;
+    return (ws_header_ints() + n) * (int64_t)sizeof(int32_t);
+}
+
+static int gjk_distance_impl(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
+                             int64_t n, const d3d_gjk_params *params, double *dist, double *cp_a,
+                             double *cp_b, int32_t *status, int32_t *iterations, double *simplex,
+                             int32_t *n_points, void *workspace, int64_t workspace_bytes,
+                             void *stream) {
+    clear_error();
+    if (int rc = check_scene(scene)) return rc;
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(n < (int64_t)0x7fffffff, "n must fit in int32");
+    D3D_CHECK_ARG(pair_a && pair_b && dist && cp_a && cp_b && status, "NULL buffer");
+    d3d_gjk_params prm;
+    if (params) {
+        prm = *params;
+    } else {
+        prm.tolerance = 1e-10;
+        prm.max_distance_squared = 100000.0;
+        prm.sanity_check = 1e-8;
+        prm.max_iterations = 128;
+        prm.hull_serial_max = 16;
+        for (int c = 0; c < kNumClasses; ++c) prm.lanes_per_pair[c] = 0;
+    }
+    D3D_CHECK_ARG(prm.max_iterations > 0, "max_iterations must be > 0");
+    for (int c = 0; c < kNumClasses; ++c) {
+        const int G = prm.lanes_per_pair[c];
+        D3D_CHECK_ARG(G == 0 || (G <= 64 && (G & (G - 1)) == 0),
+                      "lanes_per_pair entries must be 0 or a power of two <= 64");
+    }
+    void *ws = workspace;
+    if (ws) {
+        D3D_CHECK_ARG(workspace_bytes >= d3d_gjk_workspace_bytes(n), "workspace too small");
+    } else {
+        if (int rc = workspace_for(stream, n, &ws)) return rc;
+    }
+    GjkOut o{dist, cp_a, cp_b, status, iterations, simplex, n_points};
+    return launch_all(*scene, pair_a, pair_b, n, prm, o, ws, (hipStream_t)stream);
+}
+
+extern "C" int d3d_gjk_distance_batch(const d3d_scene *scene, const int32_t *pair_a,
+                                      const int32_t *pair_b, int64_t n,
+                                      const d3d_gjk_params *params, double *dist, double *cp_a,
+                                      double *cp_b, int32_t *status, int32_t *iterations,
+                                      double *simplex, int32_t *n_points, void *stream) {
+    return gjk_distance_impl(scene, pair_a, pair_b, n, params, dist, cp_a, cp_b, status, iterations,
+                             simplex, n_points, nullptr, 0, stream);
+}
+
+extern "C" int d3d_gjk_distance_batch_ws(const d3d_scene *scene, const int32_t *pair_a,
+                                         const int32_t *pair_b, int64_t n,
+                                         const d3d_gjk_params *params, double *dist, double *cp_a,
+                                         double *cp_b, int32_t *status, int32_t *iterations,
+                                         double *simplex, int32_t *n_points, void *workspace,
+                                         int64_t workspace_bytes, void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(workspace != nullptr, "workspace is NULL");
+    return gjk_distance_impl(scene, pair_a, pair_b, n, params, dist, cp_a, cp_b, status, iterations,
+                             simplex, n_points, workspace, workspace_bytes, stream);
+}

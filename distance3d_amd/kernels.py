@@ -64,12 +64,16 @@ class GjkResult:
 
 
 def gjk_distance_batch(scene, pair_a, pair_b, tolerance=1e-10, max_distance_squared=100000.0,
-                       sanity_check=1e-8, max_iterations=128, hull_serial_max=16,
-                       want_simplex=False, want_iterations=True, out=None, stream=None):
+                       sanity_check=1e-8, max_iterations=128, lanes_per_pair=None,
+                       want_simplex=False, want_iterations=True, out=None, workspace=None,
+                       stream=None):
     """GJK distance over pairs (gjk.gjk semantics per pair).
 
     ``out`` may hold preallocated tensors (dist, cp_a, cp_b, status,
-    iterations, simplex, n_points) to avoid allocation in a loop."""
+    iterations, simplex, n_points) and ``workspace`` a uint8 device tensor of
+    at least :func:`gjk_workspace_bytes` bytes, so a loop allocates nothing.
+    ``lanes_per_pair`` (8 entries, one per work class) overrides the launch
+    tuning."""
     import torch
     a = _dev_i32(pair_a, scene.device)
     b = _dev_i32(pair_b, scene.device)
@@ -87,13 +91,25 @@ def gjk_distance_batch(scene, pair_a, pair_b, tolerance=1e-10, max_distance_squa
             torch.empty((n, 3, 4, 3), dtype=torch.float64, device=dev) if want_simplex else None,
             torch.empty(n, dtype=torch.int32, device=dev) if want_simplex else None)
     prm = _native.gjk_params(tolerance, max_distance_squared, sanity_check, max_iterations,
-                             hull_serial_max)
+                             lanes_per_pair=lanes_per_pair)
     import ctypes
-    check(lib().d3d_gjk_distance_batch(
-        scene.c, ptr(a), ptr(b), n, ctypes.byref(prm), ptr(out.dist), ptr(out.cp_a),
-        ptr(out.cp_b), ptr(out.status), ptr(out.iterations), ptr(out.simplex),
-        ptr(out.n_points), stream_ptr(stream)), "d3d_gjk_distance_batch")
+    if workspace is None:
+        check(lib().d3d_gjk_distance_batch(
+            scene.c, ptr(a), ptr(b), n, ctypes.byref(prm), ptr(out.dist), ptr(out.cp_a),
+            ptr(out.cp_b), ptr(out.status), ptr(out.iterations), ptr(out.simplex),
+            ptr(out.n_points), stream_ptr(stream)), "d3d_gjk_distance_batch")
+    else:
+        check(lib().d3d_gjk_distance_batch_ws(
+            scene.c, ptr(a), ptr(b), n, ctypes.byref(prm), ptr(out.dist), ptr(out.cp_a),
+            ptr(out.cp_b), ptr(out.status), ptr(out.iterations), ptr(out.simplex),
+            ptr(out.n_points), ptr(workspace), workspace.numel(), stream_ptr(stream)),
+            "d3d_gjk_distance_batch_ws")
     return out
+
+
+def gjk_workspace_bytes(n):
+    """Device workspace bytes the batched GJK needs for n pairs."""
+    return int(_native.load_library().d3d_gjk_workspace_bytes(int(n)))
 
 
 def gjk_intersection_batch(scene, pair_a, pair_b, tolerance=1e-10, max_iterations=128,

@@ -83,15 +83,23 @@ typedef struct d3d_scene {
 } d3d_scene;
 
 /* gjk_distance_jolt keyword arguments (gjk/_gjk_jolt.py:138-139) plus the
- * iteration cap the reference loop lacks. */
+ * iteration cap the reference loop lacks and the launch tuning.
+ *
+ * Work classes: d3d_gjk_distance_batch sorts pairs by the total number of
+ * hull vertices n = n_A + n_B (0 for analytic colliders):
+ *   class 0: n = 0, 1: n <= 32, 2: <= 128, 3: <= 512, 4: <= 1024,
+ *   5: <= 2048, 6: <= 4096, 7: > 4096.
+ * lanes_per_pair[c] lanes (a power of two <= 64; 0 = built-in default
+ * 1,1,1,1,2,4,8,16) work on each pair of class c; class 0 always uses 1. */
 typedef struct d3d_gjk_params {
     double tolerance;            /* 1e-10 */
     double max_distance_squared; /* 1e5 */
     double sanity_check;         /* 1e-8 */
     int32_t max_iterations;      /* 128 */
-    int32_t hull_serial_max;     /* hulls with <= this many vertices are swept
-                                    by their own lane; larger hulls by the
-                                    whole wavefront (default 16) */
+    int32_t hull_serial_max;     /* intersection / support kernels: hulls with
+                                    <= this many vertices are swept by their
+                                    own lane, larger ones by the wavefront */
+    int32_t lanes_per_pair[8];
 } d3d_gjk_params;
 
 /* Library version / device setup. */
@@ -129,11 +137,28 @@ int d3d_aabb_batch(const d3d_scene *scene, const int32_t *shape, int64_t n, doub
  *   iterations (n) int32, optional (NULL).
  *   simplex (n,3,4,3) f64, optional: final Y, P, Q rows (the reference's
  *       returned simplex is the Y block); n_points (n) int32, optional.
+ * Uses a workspace the library keeps per (device, stream), allocated with
+ * hipMalloc on first use and grown on demand.
  */
 int d3d_gjk_distance_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
                            int64_t n, const d3d_gjk_params *params, double *dist, double *cp_a,
                            double *cp_b, int32_t *status, int32_t *iterations, double *simplex,
                            int32_t *n_points, void *stream);
+
+/* Bytes of device workspace d3d_gjk_distance_batch_ws needs for n pairs. */
+int64_t d3d_gjk_workspace_bytes(int64_t n);
+
+/*
+ * d3d_gjk_distance_batch with a caller-owned device workspace of at least
+ * d3d_gjk_workspace_bytes(n) bytes (no allocation inside; calls that overlap
+ * in time must use distinct workspaces).  Same outputs as
+ * d3d_gjk_distance_batch (gjk/_gjk_jolt.py:138-221).
+ */
+int d3d_gjk_distance_batch_ws(const d3d_scene *scene, const int32_t *pair_a,
+                              const int32_t *pair_b, int64_t n, const d3d_gjk_params *params,
+                              double *dist, double *cp_a, double *cp_b, int32_t *status,
+                              int32_t *iterations, double *simplex, int32_t *n_points,
+                              void *workspace, int64_t workspace_bytes, void *stream);
 
 /*
  * Batched GJK intersection, gjk.gjk_intersection / gjk_intersection_jolt

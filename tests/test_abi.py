@@ -15,7 +15,7 @@ def _declared():
     for fn in os.listdir(os.path.join(ROOT, "include")):
         if fn.endswith(".h"):
             src = open(os.path.join(ROOT, "include", fn)).read()
-            names |= set(re.findall(r"^(?:int|const char \*)\s*(d3d_\w+)\(", src, re.M))
+            names |= set(re.findall(r"^(?:int|int64_t|const char \*)\s*(d3d_\w+)\(", src, re.M))
     return names
 
 
