@@ -1,0 +1,495 @@
+// d3d_bvh.hip -- AABB broad phase on gfx950: linear BVH build and stackless
+// overlap queries.
+//
+// Build (all on the caller's stream, inside a caller-owned buffer):
+//   1. scene bounds of the AABB centres (block reduction)
+//   2. 30-bit Morton codes of the centres
+//   3. radix sort of (code, index), 4 passes of 8 bits (LDS histograms; the
+//      order among equal codes is arbitrary, which the hierarchy tolerates)
+//   4. Karras hierarchy: internal node i covers a range of sorted leaves and
+//      splits at the highest differing code bit (ties broken by position)
+//   5. bottom-up refit: node boxes in FP32 rounded outward, so a node box
+//      contains the exact FP64 boxes below it
+//   6. ropes: every node stores the node to visit after its subtree
+// Query: one lane per query AABB walks left child / rope without a stack;
+// FP32 node tests are conservative, and each leaf hit is confirmed with the
+// reference's closed-interval FP64 test (aabb_tree.py:503-527), so the pair
+// set equals the brute-force set of all_aabbs_overlap exactly.  Hits are
+// appended with one atomic per wavefront (ballot compaction).
+#include "d3d_device.hpp"
+#include "d3d_runtime.hpp"
+
+namespace d3d {
+
+constexpr int kBvhBlock = 256;
+constexpr int kSortItems = 16;  // items per thread in the radix sort
+constexpr int kSortTile = kBvhBlock * kSortItems;
+
+struct BvhNode {
+    float lo[3];
+    int left;
+    float hi[3];
+    int escape;
+};
+
+struct BvhHeader {
+    int64_t n;
+    double slo[3], shi[3];
+    int64_t reserved[9];
+};
+
+struct BvhView {
+    BvhHeader *hdr;
+    BvhNode *nodes;     // 2n-1: internal 0..n-2, leaves n-1..2n-2
+    int32_t *leaf_orig; // sorted position -> original index
+    int32_t *parent;    // 2n-1
+    int32_t *right;     // n-1 (refit / ropes)
+    int32_t *flags;     // n-1 refit arrival counters
+    uint32_t *keys, *keys_alt;
+    int32_t *vals_alt;
+    int32_t *hist;      // tiles * 256
+    double *bpart;      // bound partials 6 * kBoundBlocks
+};
+
+constexpr int kBoundBlocks = 256;
+
+__host__ __device__ inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+__host__ __device__ inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+__host__ __device__ inline int64_t bvh_bytes(int64_t n) {
+    const int64_t m = n > 0 ? n : 1;
+    int64_t b = align256(sizeof(BvhHeader));
+    b += align256((2 * m - 1) * (int64_t)sizeof(BvhNode));
+    b += align256(m * 4);           // leaf_orig
+    b += align256((2 * m - 1) * 4); // parent
+    b += align256(m * 4);           // right
+    b += align256(m * 4);           // flags
+    b += 2 * align256(m * 4);       // keys, keys_alt
+    b += align256(m * 4);           // vals_alt
+    b += align256(sort_tiles(m) * 256 * 4);
+    b += align256(6 * kBoundBlocks * 8);
+    return b;
+}
+
+inline BvhView bvh_view(void *base, int64_t n) {
+    const int64_t m = n > 0 ? n : 1;
+    char *p = (char *)base;
+    BvhView v;
+    v.hdr = (BvhHeader *)p;
+    p += align256(sizeof(BvhHeader));
+    v.nodes = (BvhNode *)p;
+    p += align256((2 * m - 1) * (int64_t)sizeof(BvhNode));
+    v.leaf_orig = (int32_t *)p;
+    p += align256(m * 4);
+    v.parent = (int32_t *)p;
+    p += align256((2 * m - 1) * 4);
+    v.right = (int32_t *)p;
+    p += align256(m * 4);
+    v.flags = (int32_t *)p;
+    p += align256(m * 4);
+    v.keys = (uint32_t *)p;
+    p += align256(m * 4);
+    v.keys_alt = (uint32_t *)p;
+    p += align256(m * 4);
+    v.vals_alt = (int32_t *)p;
+    p += align256(m * 4);
+    v.hist = (int32_t *)p;
+    p += align256(sort_tiles(m) * 256 * 4);
+    v.bpart = (double *)p;
+    return v;
+}
+
+// ------------------------------------------------------------- bounds
+
+__global__ __launch_bounds__(kBvhBlock) void bvh_bounds_partial(const double *__restrict__ aabbs,
+                                                                int64_t n, double *__restrict__ part) {
+    __shared__ double red[6][kBvhBlock];
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBvhBlock) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double c = (aabbs[6 * i + 2 * a] + aabbs[6 * i + 2 * a + 1]) * 0.5;
+            lo[a] = fmin(lo[a], c);
+            hi[a] = fmax(hi[a], c);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = lo[a];
+        red[3 + a][threadIdx.x] = hi[a];
+    }
+    __syncthreads();
+    for (int off = kBvhBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                red[a][threadIdx.x] = fmin(red[a][threadIdx.x], red[a][threadIdx.x + off]);
+                red[3 + a][threadIdx.x] = fmax(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + off]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(64) void bvh_bounds_final(const double *__restrict__ part, int nparts,
+                                                       BvhHeader *__restrict__ hdr, int64_t n) {
+    if (threadIdx.x == 0) {
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int b = 0; b < nparts; ++b)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fmin(lo[a], part[6 * b + a]);
+                hi[a] = fmax(hi[a], part[6 * b + 3 + a]);
+            }
+        hdr->n = n;
+        for (int a = 0; a < 3; ++a) {
+            hdr->slo[a] = lo[a];
+            hdr->shi[a] = hi[a];
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(kBvhBlock) void bvh_morton_kernel(const double *__restrict__ aabbs,
+                                                               int64_t n,
+                                                               const BvhHeader *__restrict__ hdr,
+                                                               uint32_t *__restrict__ keys,
+                                                               int32_t *__restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double c = (aabbs[6 * i + 2 * a] + aabbs[6 * i + 2 * a + 1]) * 0.5;
+        const double ext = hdr->shi[a] - hdr->slo[a];
+        double t = ext > 0.0 ? (c - hdr->slo[a]) / ext : 0.0;
+        t = fmin(fmax(t, 0.0), 1.0);
+        q[a] = (uint32_t)fmin(t * 1024.0, 1023.0);
+    }
+    keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    vals[i] = (int32_t)i;
+}
+
+// --------------------------------------------------------- radix sort
+
+__global__ __launch_bounds__(kBvhBlock) void rs_hist_kernel(const uint32_t *__restrict__ keys,
+                                                            int64_t n, int shift,
+                                                            int32_t *__restrict__ hist) {
+    __shared__ int32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    for (int j = 0; j < kSortItems; ++j) {
+        const int64_t i = base + (int64_t)j * kBvhBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    // digit-major layout: hist[digit * tiles + tile]
+    hist[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of hist (digit-major) in place, one block
+__global__ __launch_bounds__(1024) void rs_scan_kernel(int32_t *__restrict__ hist, int64_t len) {
+    __shared__ int32_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (len + 1023) / 1024;
+    const int64_t b0 = t * per, b1 = (b0 + per < len) ? b0 + per : len;
+    int32_t s = 0;
+    for (int64_t i = b0; i < b1; ++i) s += hist[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = part[t] - s;
+    for (int64_t i = b0; i < b1; ++i) {
+        const int32_t c = hist[i];
+        hist[i] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(kBvhBlock) void rs_scatter_kernel(
+    const uint32_t *__restrict__ keys, const int32_t *__restrict__ vals, int64_t n, int shift,
+    const int32_t *__restrict__ offs, uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out) {
+    __shared__ int32_t h[256];
+    h[threadIdx.x] = offs[(int64_t)threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    for (int j = 0; j < kSortItems; ++j) {
+        const int64_t i = base + (int64_t)j * kBvhBlock + threadIdx.x;
+        if (i < n) {
+            const uint32_t k = keys[i];
+            const int pos = atomicAdd(&h[(k >> shift) & 255u], 1);
+            keys_out[pos] = k;
+            vals_out[pos] = vals[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------- hierarchy
+
+__device__ __forceinline__ int delta(const uint32_t *__restrict__ k, int64_t n, int i, int j) {
+    if (j < 0 || (int64_t)j >= n) return -1;
+    const uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((unsigned)(i ^ j));
+    return __clz(a ^ b);
+}
+
+__global__ __launch_bounds__(kBvhBlock) void bvh_hierarchy_kernel(const uint32_t *__restrict__ keys,
+                                                                  int64_t n,
+                                                                  BvhNode *__restrict__ nodes,
+                                                                  int32_t *__restrict__ parent,
+                                                                  int32_t *__restrict__ right,
+                                                                  int32_t *__restrict__ flags) {
+    const int64_t ii = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
+    if (ii >= n - 1) return;
+    const int i = (int)ii;
+    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int leaf0 = (int)(n - 1);
+    const int lc = (lo == gamma) ? leaf0 + gamma : gamma;
+    const int rc = (hi == gamma + 1) ? leaf0 + gamma + 1 : gamma + 1;
+    nodes[i].left = lc;
+    right[i] = rc;
+    parent[lc] = i;
+    parent[rc] = i;
+    flags[i] = 0;
+    if (i == 0) parent[0] = -1;
+}
+
+__device__ __forceinline__ void store_box(BvhNode *nd, const float lo[3], const float hi[3]) {
+    nd->lo[0] = lo[0];
+    nd->lo[1] = lo[1];
+    nd->lo[2] = lo[2];
+    nd->hi[0] = hi[0];
+    nd->hi[1] = hi[1];
+    nd->hi[2] = hi[2];
+}
+
+// leaves get their outward-rounded FP32 box, then climb: the second child
+// to arrive at a node computes its box (union of the children).
+__global__ __launch_bounds__(kBvhBlock) void bvh_refit_kernel(const double *__restrict__ aabbs,
+                                                              int64_t n,
+                                                              const int32_t *__restrict__ sorted_vals,
+                                                              int32_t *__restrict__ leaf_orig,
+                                                              BvhNode *nodes, const int32_t *parent,
+                                                              const int32_t *right, int32_t *flags) {
+    const int64_t i = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
+    if (i >= n) return;
+    const int orig = sorted_vals[i];
+    leaf_orig[i] = orig;
+    float lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = __double2float_rd(aabbs[6 * (int64_t)orig + 2 * a]);
+        hi[a] = __double2float_ru(aabbs[6 * (int64_t)orig + 2 * a + 1]);
+    }
+    const int leaf = (int)(n - 1 + i);
+    store_box(&nodes[leaf], lo, hi);
+    nodes[leaf].left = -1;
+    if (n == 1) return;
+    __threadfence();
+    int p = parent[leaf];
+    while (p >= 0) {
+        if (atomicAdd(&flags[p], 1) == 0) return;
+        __threadfence();
+        const int a = nodes[p].left, b = right[p];
+        const volatile BvhNode *na = &nodes[a];
+        const volatile BvhNode *nb = &nodes[b];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(na->lo[k], nb->lo[k]);
+            hi[k] = fmaxf(na->hi[k], nb->hi[k]);
+        }
+        store_box(&nodes[p], lo, hi);
+        __threadfence();
+        p = parent[p];
+    }
+}
+
+__global__ __launch_bounds__(kBvhBlock) void bvh_rope_kernel(int64_t n, BvhNode *__restrict__ nodes,
+                                                             const int32_t *__restrict__ parent,
+                                                             const int32_t *__restrict__ right) {
+    const int64_t k = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
+    if (k >= 2 * n - 1) return;
+    int cur = (int)k, esc = -1;
+    while (cur != 0) {
+        const int p = parent[cur];
+        if (nodes[p].left == cur) {
+            esc = right[p];
+            break;
+        }
+        cur = p;
+    }
+    nodes[k].escape = esc;
+}
+
+// ------------------------------------------------------------- query
+
+__global__ __launch_bounds__(kBvhBlock) void bvh_query_kernel(
+    const BvhNode *__restrict__ nodes, const int32_t *__restrict__ leaf_orig, int64_t n,
+    const double *__restrict__ aabbs, const double *__restrict__ q, int64_t m,
+    int32_t *__restrict__ pairs, int64_t cap, unsigned long long *__restrict__ counter) {
+    const int64_t qi = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
+    if (qi >= m) return;
+    double Q[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Q[k] = q[6 * qi + k];
+    float qlo[3], qhi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        qlo[a] = __double2float_rd(Q[2 * a]);
+        qhi[a] = __double2float_ru(Q[2 * a + 1]);
+    }
+    const int lane = lane_id();
+    const int leaf0 = (int)(n - 1);
+    int node = 0;
+    // a rope walk visits each node at most once; the cap only guards
+    // against a corrupted tree
+    for (int64_t steps = 0; node >= 0 && steps < 2 * n; ++steps) {
+        const BvhNode nd = nodes[node];
+        const bool ov = nd.lo[0] <= qhi[0] && nd.hi[0] >= qlo[0] && nd.lo[1] <= qhi[1] &&
+                        nd.hi[1] >= qlo[1] && nd.lo[2] <= qhi[2] && nd.hi[2] >= qlo[2];
+        bool hit = false;
+        int orig = 0;
+        if (ov && node >= leaf0) {
+            orig = leaf_orig[node - leaf0];
+            const double *A = aabbs + 6 * (int64_t)orig;
+            // aabb_overlap(tree_aabb, test_aabb), aabb_tree.py:520-527
+            hit = A[0] <= Q[1] && A[1] >= Q[0] && A[2] <= Q[3] && A[3] >= Q[2] && A[4] <= Q[5] &&
+                  A[5] >= Q[4];
+        }
+        const unsigned long long msk = __ballot(hit);
+        if (hit) {
+            const int first = __ffsll((long long)msk) - 1;
+            unsigned long long base = 0;
+            if (lane == first) base = atomicAdd(counter, (unsigned long long)__popcll(msk));
+            const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)base, first);
+            const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(base >> 32), first);
+            const unsigned long long pos =
+                (((unsigned long long)hi << 32) | lo) +
+                (unsigned long long)__popcll(msk & (lane == 0 ? 0ull : (~0ull >> (64 - lane))));
+            if (pos < (unsigned long long)cap) {
+                pairs[2 * pos] = (int32_t)qi;
+                pairs[2 * pos + 1] = orig;
+            }
+        }
+        node = (ov && node < leaf0) ? nd.left : nd.escape;
+    }
+}
+
+__global__ void bvh_zero_counter(unsigned long long *c) { *c = 0ull; }
+
+}  // namespace d3d
+
+using namespace d3d;
+
+extern "C" int64_t d3d_bvh_bytes(int64_t n) {
+    if (n < 0) return -1;
+    return bvh_bytes(n);
+}
+
+extern "C" int d3d_bvh_build(const double *aabbs, int64_t n, void *bvh, int64_t bvh_bytes_avail,
+                             void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(n >= 1 && n < (int64_t)0x40000000, "n must be in [1, 2^30)");
+    D3D_CHECK_ARG(aabbs && bvh, "NULL buffer");
+    D3D_CHECK_ARG(bvh_bytes_avail >= bvh_bytes(n), "bvh buffer smaller than d3d_bvh_bytes(n)");
+    hipStream_t st = (hipStream_t)stream;
+    BvhView v = bvh_view(bvh, n);
+    int nb = (int)((n + kBvhBlock - 1) / kBvhBlock);
+    const int nparts = nb < kBoundBlocks ? nb : kBoundBlocks;
+    hipLaunchKernelGGL(bvh_bounds_partial, dim3(nparts), dim3(kBvhBlock), 0, st, aabbs, n, v.bpart);
+    D3D_CHECK_LAUNCH("bvh_bounds_partial");
+    hipLaunchKernelGGL(bvh_bounds_final, dim3(1), dim3(64), 0, st, v.bpart, nparts, v.hdr, n);
+    D3D_CHECK_LAUNCH("bvh_bounds_final");
+    // codes into keys_alt/vals_alt, sorted back and forth; 4 passes end in keys/leaf_orig-source
+    uint32_t *kin = v.keys, *kout = v.keys_alt;
+    int32_t *vin = v.leaf_orig, *vout = v.vals_alt;
+    hipLaunchKernelGGL(bvh_morton_kernel, dim3(nb), dim3(kBvhBlock), 0, st, aabbs, n, v.hdr, kin, vin);
+    D3D_CHECK_LAUNCH("bvh_morton_kernel");
+    const int64_t tiles = sort_tiles(n);
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(rs_hist_kernel, dim3((unsigned)tiles), dim3(kBvhBlock), 0, st, kin, n,
+                           shift, v.hist);
+        D3D_CHECK_LAUNCH("rs_hist_kernel");
+        hipLaunchKernelGGL(rs_scan_kernel, dim3(1), dim3(1024), 0, st, v.hist, tiles * 256);
+        D3D_CHECK_LAUNCH("rs_scan_kernel");
+        hipLaunchKernelGGL(rs_scatter_kernel, dim3((unsigned)tiles), dim3(kBvhBlock), 0, st, kin, vin,
+                           n, shift, v.hist, kout, vout);
+        D3D_CHECK_LAUNCH("rs_scatter_kernel");
+        uint32_t *tk = kin; kin = kout; kout = tk;
+        int32_t *tv = vin; vin = vout; vout = tv;
+    }
+    // after 4 passes the sorted data is back in keys / leaf_orig's buffer;
+    // the refit copies the permutation into leaf_orig (same buffer: no-op copy)
+    if (n > 1) {
+        const int ni = (int)((n - 1 + kBvhBlock - 1) / kBvhBlock);
+        hipLaunchKernelGGL(bvh_hierarchy_kernel, dim3(ni), dim3(kBvhBlock), 0, st, kin, n, v.nodes,
+                           v.parent, v.right, v.flags);
+        D3D_CHECK_LAUNCH("bvh_hierarchy_kernel");
+    }
+    hipLaunchKernelGGL(bvh_refit_kernel, dim3(nb), dim3(kBvhBlock), 0, st, aabbs, n, vin, v.leaf_orig,
+                       v.nodes, v.parent, v.right, v.flags);
+    D3D_CHECK_LAUNCH("bvh_refit_kernel");
+    if (n > 1) {
+        const int nn = (int)((2 * n - 1 + kBvhBlock - 1) / kBvhBlock);
+        hipLaunchKernelGGL(bvh_rope_kernel, dim3(nn), dim3(kBvhBlock), 0, st, n, v.nodes, v.parent,
+                           v.right);
+        D3D_CHECK_LAUNCH("bvh_rope_kernel");
+    } else {
+        hipLaunchKernelGGL(bvh_rope_kernel, dim3(1), dim3(kBvhBlock), 0, st, n, v.nodes, v.parent,
+                           v.right);
+        D3D_CHECK_LAUNCH("bvh_rope_kernel");
+    }
+    return D3D_OK;
+}
+
+extern "C" int d3d_bvh_query(const void *bvh, int64_t n, const double *aabbs, const double *queries,
+                             int64_t m, int32_t *pairs, int64_t cap, uint64_t *n_pairs,
+                             void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(n >= 1 && n < (int64_t)0x40000000, "n must be in [1, 2^30)");
+    D3D_CHECK_ARG(m >= 0 && m < (int64_t)0x7fffffff, "m must fit in int32");
+    D3D_CHECK_ARG(bvh && aabbs && n_pairs && (m == 0 || queries), "NULL buffer");
+    D3D_CHECK_ARG(cap >= 0 && (cap == 0 || pairs), "pairs buffer is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(bvh_zero_counter, dim3(1), dim3(1), 0, st, (unsigned long long *)n_pairs);
+    D3D_CHECK_LAUNCH("bvh_zero_counter");
+    if (m == 0) return D3D_OK;
+    BvhView v = bvh_view((void *)bvh, n);
+    const int nb = (int)((m + kBvhBlock - 1) / kBvhBlock);
+    hipLaunchKernelGGL(bvh_query_kernel, dim3(nb), dim3(kBvhBlock), 0, st, v.nodes, v.leaf_orig, n,
+                       aabbs, queries, m, pairs, cap, (unsigned long long *)n_pairs);
+    D3D_CHECK_LAUNCH("bvh_query_kernel");
+    return D3D_OK;
+}

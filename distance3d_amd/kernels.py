@@ -166,3 +166,59 @@ def epa_batch(scene, pair_a, pair_b, simplex, n_points, max_iterations=64, max_f
                               int(max_iterations), int(max_faces), float(epsilon), ptr(out),
                               ptr(status), ptr(iters), stream_ptr(stream)), "d3d_epa_batch")
     return EpaResult(out, status, iters)
+
+
+class Bvh:
+    """A linear BVH over ``aabbs`` (n,3,2) float64 device tensor, held in a
+    device buffer (uint8 tensor).  The boxes must stay unchanged while the
+    BVH is queried."""
+
+    def __init__(self, aabbs, buffer, n):
+        self.aabbs = aabbs
+        self.buffer = buffer
+        self.n = n
+
+
+def bvh_build(aabbs, stream=None):
+    """Build a BVH over (n,3,2) float64 boxes (tensor or array)."""
+    import torch
+    a = torch.as_tensor(aabbs, dtype=torch.float64)
+    a = a.cuda() if not a.is_cuda else a
+    a = a.reshape(-1, 3, 2).contiguous()
+    n = a.shape[0]
+    if n < 1:
+        raise ValueError("a BVH needs at least one box")
+    buf = torch.empty(int(_native.load_library().d3d_bvh_bytes(n)), dtype=torch.uint8,
+                      device=a.device)
+    check(lib().d3d_bvh_build(ptr(a), n, ptr(buf), buf.numel(), stream_ptr(stream)),
+          "d3d_bvh_build")
+    return Bvh(a, buf, n)
+
+
+def bvh_query(bvh, queries, cap=None, counter=None, stream=None, sort=True):
+    """All (query, box) index pairs with closed-interval overlap, as an
+    (k, 2) int32 device tensor (lexicographically sorted when ``sort``).
+
+    Reads the pair count back to size the output (one host synchronisation);
+    a second launch runs when ``cap`` was too small."""
+    import torch
+    q = torch.as_tensor(queries, dtype=torch.float64)
+    q = q.to(bvh.aabbs.device).reshape(-1, 3, 2).contiguous()
+    m = q.shape[0]
+    if counter is None:
+        counter = torch.zeros(1, dtype=torch.int64, device=q.device)
+    if cap is None:
+        cap = 4 * m + 1024
+    while True:
+        pairs = torch.empty((max(cap, 1), 2), dtype=torch.int32, device=q.device)
+        check(lib().d3d_bvh_query(ptr(bvh.buffer), bvh.n, ptr(bvh.aabbs), ptr(q), m, ptr(pairs),
+                                  cap, ptr(counter), stream_ptr(stream)), "d3d_bvh_query")
+        k = int(counter.item())
+        if k <= cap:
+            break
+        cap = k
+    pairs = pairs[:k]
+    if sort and k > 1:
+        key = pairs[:, 0].to(torch.int64) * bvh.n + pairs[:, 1].to(torch.int64)
+        pairs = pairs[torch.argsort(key)]
+    return pairs

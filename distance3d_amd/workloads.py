@@ -13,6 +13,10 @@ FP64 pose is the geometry both the GPU and the FP64 checker see.
 * ``epa_pairs``    -- 262,144 overlapping hull pairs, 64..256 vertices on
   ellipsoid surfaces (radii U(0.3, 1)), projections along a random axis
   overlapping by U(0.01, 0.5), seed 2.
+* ``broad_phase_scene`` -- 1,048,576 oriented boxes (sizes log-uniform
+  0.01..0.5 per axis) and 65,536 capsules (r U(0.05, 0.5), h U(0.1, 2)) with
+  centres uniform in [0, 100]^3, seed 3; boxes are shapes 0..n-1, capsules
+  follow.
 """
 import numpy as np
 
@@ -191,3 +195,23 @@ def epa_pairs(seed=2, n_pairs=262144, min_vertices=64, max_vertices=256):
     scene = _scene_from(types, R, t, np.zeros((n, 3)), counts, v32)
     return Workload("epa_pairs", scene, np.arange(0, n, 2), np.arange(1, n, 2),
                     {"seed": seed, "n_pairs": n_pairs, "delta": delta, "axis": axis})
+
+
+def broad_phase_scene(seed=3, n_boxes=1 << 20, n_capsules=65536, extent=100.0):
+    """Boxes and query capsules for the AABB broad phase."""
+    rng = np.random.default_rng(seed)
+    n = n_boxes + n_capsules
+    types = np.empty(n, dtype=np.int32)
+    types[:n_boxes] = _native.BOX
+    types[n_boxes:] = _native.CAPSULE
+    R = uniform_rotations(rng, n)
+    t = rng.uniform(0.0, extent, (n, 3))
+    params = np.zeros((n, 3))
+    params[:n_boxes] = np.exp(rng.uniform(np.log(0.01), np.log(0.5), (n_boxes, 3)))
+    params[n_boxes:, 0] = rng.uniform(0.05, 0.5, n_capsules)
+    params[n_boxes:, 1] = rng.uniform(0.1, 2.0, n_capsules)
+    scene = _scene_from(types, R, t, params, np.zeros(n, dtype=np.int64),
+                        np.zeros((0, 3), dtype=np.float32))
+    return Workload("broad_phase_scene", scene, np.zeros(0, dtype=np.int32),
+                    np.zeros(0, dtype=np.int32),
+                    {"seed": seed, "n_boxes": n_boxes, "n_capsules": n_capsules})

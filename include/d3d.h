@@ -189,6 +189,25 @@ int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *
                   int32_t max_iterations, int32_t max_faces, double epsilon, double *out,
                   int32_t *status, int32_t *iterations, void *stream);
 
+/*
+ * Broad phase: linear BVH over n AABBs and overlap queries
+ * (aabb_tree.py:15-191 AabbTree, 381-403 query_overlap, 465-527
+ * all_aabbs_overlap / aabb_overlap).
+ *
+ * d3d_bvh_bytes(n): size of the caller-owned device buffer holding a BVH
+ * over n boxes.  d3d_bvh_build(aabbs (n,3,2) f64 [axis][min,max], n, bvh,
+ * bytes, stream) builds it (the aabbs must stay alive and unchanged while
+ * the BVH is queried).  d3d_bvh_query(bvh, n, aabbs, queries (m,3,2) f64, m,
+ * pairs (cap,2) int32, cap, n_pairs, stream) writes every (query index,
+ * box index) with aabb_overlap(box, query) true -- closed intervals on all
+ * three axes, FP64 -- in arbitrary order, and the total count to the device
+ * uint64 *n_pairs (pairs beyond cap are counted but not written).
+ */
+int64_t d3d_bvh_bytes(int64_t n);
+int d3d_bvh_build(const double *aabbs, int64_t n, void *bvh, int64_t bvh_bytes, void *stream);
+int d3d_bvh_query(const void *bvh, int64_t n, const double *aabbs, const double *queries,
+                  int64_t m, int32_t *pairs, int64_t cap, uint64_t *n_pairs, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
