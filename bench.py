@@ -1,14 +1,18 @@
-"""Benchmark: batched GJK pair-queries/s on one MI355X.
+"""Benchmark: batched GJK+EPA pair-queries/s on one MI355X.
 
-Workload: 1,048,576 mixed-primitive pairs (sphere / capsule / cylinder / box /
-ellipsoid / convex hull with log-uniform 8..4096 vertices), seed 1, scene and
-pairs resident in HBM before the timed region.  A step is one pass of the
-batched GJK distance kernel over all pairs.
+Headline workload: 1,048,576 mixed-primitive pairs (sphere / capsule /
+cylinder / box / ellipsoid / convex hull with log-uniform 8..4096 vertices),
+seed 1, scene and pairs resident in HBM before the timed region.  A step is
+one pass of the GJK+EPA pipeline over all pairs: GJK distance and closest
+points for every pair, EPA depth / normal / contact points for the
+intersecting ones.
 
-Prints ONE JSON line with throughput, the roofline of the GJK kernel
-(algorithmic bytes per launch / average launch time, against 8 TB/s), the
-parity of the last step against the FP64 oracle, and the oracle timed on the
-host cores as the CPU baseline.
+The other synthetic configurations are measured into the same JSON line
+("configs"): 4096 hull pairs, 262,144 EPA pairs, the broad phase over
+1,048,576 boxes with 65,536 capsule queries plus GJK on the candidates, and
+the continuous-batching server on a Poisson request stream.  Each carries its
+parity against the FP64 oracle.  The oracle timed on the host cores is the
+CPU baseline.
 """
 import argparse
 import json
@@ -23,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "GJK+EPA pair-queries/s at ≤1e-5 rel err vs FP64; p50/p99 request latency"
+MAXF = np.finfo(float).max
 
 
 def parse():
@@ -35,13 +40,229 @@ def parse():
     p.add_argument("--lanes", type=str, default=None,
                    help="lanes per pair for the 8 work classes, comma separated")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--no-parity", action="store_true", help="skip the oracle parity check")
+    p.add_argument("--no-parity", action="store_true", help="skip the oracle parity checks")
+    p.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    p.add_argument("--only", type=str, default=None,
+                   help="comma list of extra configs: hull,epa,broad,serving")
     p.add_argument("--cpu-runs", type=int, default=3)
+    p.add_argument("--serving-requests", type=int, default=50000)
     return p.parse_args()
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def timed_steps(step, steps, warmup):
+    """Runs warmup + timed steps; returns (wall ms/step, event ms per step)."""
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        step()
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, np.array([a.elapsed_time(b) for a, b in ev])
+
+
+def rel_err_stats(gpu, ref):
+    sep = (ref > 1e-9) & (ref < MAXF)
+    rel = np.abs(gpu[sep] - ref[sep]) / ref[sep]
+    band = (ref < 1e-9) | (gpu < 1e-9)
+    return {
+        "max_rel_dist_err": float(rel.max(initial=0.0)),
+        "p99_rel_dist_err": float(np.percentile(rel, 99)) if len(rel) else 0.0,
+        "bit_identical_frac": float(np.mean(gpu == ref)),
+        "flag_mismatches_outside_1e-9_band": int(np.sum(((gpu == 0.0) != (ref == 0.0)) & ~band)),
+        "intersecting_frac": float(np.mean(ref == 0.0)),
+    }
+
+
+def epa_parity(gpu_out, gpu_status, ref):
+    conv = ref["status"] == 1
+    d, od = gpu_out[:, 0], ref["depth"]
+    rel = np.abs(d[conv] - od[conv]) / np.maximum(np.abs(od[conv]), 1e-300)
+    return {
+        "pairs": int(len(od)),
+        "status_equal_frac": float(np.mean(gpu_status == ref["status"])),
+        "converged_frac": float(np.mean(conv)),
+        "max_rel_depth_err": float(rel.max(initial=0.0)),
+        "p99_rel_depth_err": float(np.percentile(rel, 99)) if len(rel) else 0.0,
+        "max_normal_err": float(np.abs(gpu_out[conv, 1:4] - ref["normal"][conv]).max(initial=0.0)),
+        "max_contact_err": float(max(
+            np.abs(gpu_out[conv, 7:10] - ref["cp_a"][conv]).max(initial=0.0),
+            np.abs(gpu_out[conv, 10:13] - ref["cp_b"][conv]).max(initial=0.0))),
+        "bit_identical_depth_frac": float(np.mean(d == od)),
+    }
+
+
+def run_pipeline_config(w, args, lanes, steps, warmup, label, parity=True, cpu=False):
+    """GJK+EPA pipeline over a workload: timing, roofline, parity, CPU."""
+    import torch
+    from distance3d_amd import kernels
+    dev = w.scene.to_device("cuda:0")
+    pa = torch.from_numpy(w.pair_a).cuda()
+    pb = torch.from_numpy(w.pair_b).cuda()
+    n = w.n_pairs
+    out = kernels.GjkEpaResult.empty(n, "cuda")
+    ws = torch.empty(kernels.gjk_epa_workspace_bytes(n), dtype=torch.uint8, device="cuda")
+
+    def step():
+        kernels.gjk_epa_batch(dev, pa, pb, out=out, workspace=ws, lanes_per_pair=lanes)
+
+    wall, ev_ms = timed_steps(step, steps, warmup)
+    value = n / (wall / 1e3)
+    dist = out.dist.cpu().numpy()
+    hit = dist == 0.0
+    # algorithmic bytes per pair: 12 (n_A + n_B) FP32 hull bytes, 2 x 64 pose
+    # and parameters, 8 pair ids, 60 GJK outputs, +80 EPA outputs if hit
+    bytes_step = float(np.sum(w.hull_bytes_per_pair()) + n * (128 + 8 + 60) + 80 * hit.sum())
+    achieved = bytes_step / (ev_ms.mean() / 1e3) / 1e9
+    res = {"workload": label, "pairs": n, "value": round(value, 1), "unit": "pair-queries/s",
+           "ms_per_step": round(wall, 4), "event_ms_per_step": round(float(ev_ms.mean()), 4),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "bytes_per_launch": bytes_step}}
+    if parity:
+        from oracle import oracle
+        t0 = time.time()
+        o = oracle.gjk_distance(w.scene, w.pair_a, w.pair_b, want_simplex=True)
+        res["parity"] = rel_err_stats(dist, o["dist"])
+        idx = np.nonzero(o["dist"] == 0.0)[0]
+        if len(idx):
+            e = oracle.epa(w.scene, w.pair_a[idx], w.pair_b[idx], o["simplex"][idx],
+                           o["n_points"][idx])
+            res["parity"]["epa"] = epa_parity(out.epa_out.cpu().numpy()[idx],
+                                              out.epa_status.cpu().numpy()[idx], e)
+        log("%s parity in %.1fs" % (label, time.time() - t0))
+    if cpu:
+        from oracle import oracle
+        cores = oracle.num_threads()
+
+        def cpu_step():
+            o = oracle.gjk_distance(w.scene, w.pair_a, w.pair_b, want_simplex=True)
+            idx = np.nonzero(o["dist"] == 0.0)[0]
+            if len(idx):
+                oracle.epa(w.scene, w.pair_a[idx], w.pair_b[idx], o["simplex"][idx],
+                           o["n_points"][idx])
+        sub = min(n, 65536)
+        oracle.gjk_distance(w.scene, w.pair_a[:sub], w.pair_b[:sub])
+        times = []
+        for _ in range(args.cpu_runs):
+            t0 = time.perf_counter()
+            cpu_step()
+            times.append(time.perf_counter() - t0)
+        rate = n / float(np.median(times))
+        res["cpu_baseline"] = {
+            "value": round(rate, 1), "unit": "pair-queries/s", "cores": cores, "kind": "port",
+            "sample": "all %d pairs of the same workload (GJK + EPA on intersecting pairs), "
+                      "median of %d runs, %.1fs CPU wall in total; C FP64 restatement, OpenMP"
+                      % (n, args.cpu_runs, sum(times))}
+    return res
+
+
+def run_broad(args):
+    import torch
+    from distance3d_amd import kernels, workloads
+    t0 = time.time()
+    w = workloads.broad_phase_scene()
+    nb = w.meta["n_boxes"]
+    dev = w.scene.to_device("cuda:0")
+    log("broad phase workload in %.1fs" % (time.time() - t0))
+    ids = torch.arange(w.scene.n_shapes, dtype=torch.int32, device="cuda")
+    aabbs = kernels.aabb_batch(dev, ids)
+    buf = torch.empty(int(kernels._native.load_library().d3d_bvh_bytes(nb)), dtype=torch.uint8,
+                      device="cuda")
+    cap = 1 << 21
+    pairs = torch.empty((cap, 2), dtype=torch.int32, device="cuda")
+    counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+    lib = kernels.lib()
+    P = kernels.ptr
+    box_ptr = aabbs.data_ptr()
+    q_ptr = aabbs[nb:].data_ptr()
+
+    def build():
+        kernels.check(lib.d3d_aabb_batch(dev.c, P(ids), w.scene.n_shapes, P(aabbs),
+                                         kernels.stream_ptr()), "aabb")
+        kernels.check(lib.d3d_bvh_build(box_ptr, nb, P(buf), buf.numel(), kernels.stream_ptr()),
+                      "build")
+
+    def query():
+        kernels.check(lib.d3d_bvh_query(P(buf), nb, box_ptr, q_ptr, w.scene.n_shapes - nb,
+                                        P(pairs), cap, P(counter), kernels.stream_ptr()), "query")
+
+    b_wall, b_ev = timed_steps(build, 10, 2)
+    q_wall, q_ev = timed_steps(query, 10, 2)
+    k = int(counter.item())
+    cand = pairs[:k].cpu().numpy()
+    ca = (nb + cand[:, 0]).astype(np.int32)
+    cb = cand[:, 1].astype(np.int32)
+    wn = workloads.Workload("broad_phase_candidates", w.scene, ca, cb)
+    narrow = run_pipeline_config(wn, args, None, 10, 2, "broad-phase candidates (capsule x box)",
+                                 parity=not args.no_parity)
+    # leaf AABBs 48 B + queries 48 B + 8 B per emitted pair
+    q_bytes = nb * 48 + (w.scene.n_shapes - nb) * 48 + 8 * k
+    res = {"workload": "broad_phase (1,048,576 boxes, 65,536 capsule queries)",
+           "boxes": nb, "queries": w.scene.n_shapes - nb, "candidate_pairs": k,
+           "build_ms": round(float(b_ev.mean()), 4), "query_ms": round(float(q_ev.mean()), 4),
+           "query_roofline": {"bound": "hbm", "achieved": round(q_bytes / (q_ev.mean() / 1e3) / 1e9, 2),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+           "narrow_phase": narrow}
+    if not args.no_parity:
+        from oracle import oracle
+        t0 = time.time()
+        host = aabbs.cpu().numpy()
+        o, cnt = oracle.overlap_pairs_sorted(host[nb:], host[:nb], cap=cap)
+        g = cand[np.lexsort((cand[:, 1], cand[:, 0]))].astype(np.int64)
+        o = o[np.lexsort((o[:, 1], o[:, 0]))]
+        res["parity"] = {"pair_set_equal": bool(cnt == k and np.array_equal(g, o)),
+                         "oracle_pairs": int(cnt),
+                         "aabbs_bitwise_equal": bool(np.array_equal(
+                             host, oracle.aabbs(w.scene, np.arange(w.scene.n_shapes))))}
+        log("broad parity in %.1fs" % (time.time() - t0))
+    return res
+
+
+def run_serving(args, lanes):
+    from distance3d_amd import serving, workloads
+    t0 = time.time()
+    scene, poses, reqs = workloads.serving_workload(n_requests=args.serving_requests)
+    log("serving workload in %.1fs: %d requests, %d pairs"
+        % (time.time() - t0, len(reqs), sum(r.n_pairs for r in reqs)))
+    dev = scene.to_device("cuda:0")
+    srv = serving.CollisionServer(dev, poses, n_slots=3, slot_pairs=16384,
+                                  lanes_per_pair=lanes)
+    log("server ready (graphs captured)")
+    res, stats = srv.serve(reqs, realtime=True)
+    stats["workload"] = ("Poisson stream, %d requests at 20k req/s, log-uniform 1..4096 "
+                         "pose-updated pairs, 512-collider resident scene" % len(reqs))
+    stats["slot_bytes"] = serving.CollisionServer.slot_bytes(16384)
+    if not args.no_parity:
+        from oracle import oracle
+        worst = 0.0
+        flags = 0
+        checked = 0
+        for r in reqs[:: max(1, len(reqs) // 200)]:
+            host = workloads.posed_instances(scene, poses, r.pair_a, r.pair_b, r.pose_a, r.pose_b)
+            k = r.n_pairs
+            g = oracle.gjk_distance(host, np.arange(0, 2 * k, 2), np.arange(1, 2 * k, 2))
+            d = res[r.rid].dist
+            flags += int(np.sum((d == 0.0) != (g["dist"] == 0.0)))
+            sep = g["dist"] > 1e-9
+            if sep.any():
+                worst = max(worst, float(np.max(np.abs(d[sep] - g["dist"][sep]) / g["dist"][sep])))
+            checked += k
+        stats["parity"] = {"pairs_checked": checked, "max_rel_dist_err": worst,
+                           "flag_mismatches": flags}
+    return stats
 
 
 def main():
@@ -51,119 +272,62 @@ def main():
               "with no exchange, so a multi-GPU figure is N times this one and is not "
               "measured here (requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    import torch
-    from distance3d_amd import kernels, workloads
+    import torch  # noqa: F401
+    from distance3d_amd import _native, workloads
 
+    lanes = [int(x) for x in args.lanes.split(",")] if args.lanes else None
     t0 = time.time()
     w = workloads.mixed_pairs(seed=args.seed, n_pairs=args.pairs)
     log("workload generated in %.1fs: %d pairs, %d hull vertices"
         % (time.time() - t0, w.n_pairs, w.scene.n_vertices))
-    dev = w.scene.to_device("cuda:0")
-    pa = torch.from_numpy(w.pair_a).cuda()
-    pb = torch.from_numpy(w.pair_b).cuda()
-    n = w.n_pairs
-    out = kernels.GjkResult(
-        torch.empty(n, dtype=torch.float64, device="cuda"),
-        torch.empty((n, 3), dtype=torch.float64, device="cuda"),
-        torch.empty((n, 3), dtype=torch.float64, device="cuda"),
-        torch.empty(n, dtype=torch.int32, device="cuda"),
-        torch.empty(n, dtype=torch.int32, device="cuda"), None, None)
-
-    lanes = [int(x) for x in args.lanes.split(",")] if args.lanes else None
-    ws = torch.empty(kernels.gjk_workspace_bytes(n), dtype=torch.uint8, device="cuda")
-
-    def step():
-        kernels.gjk_distance_batch(dev, pa, pb, lanes_per_pair=lanes, out=out, workspace=ws)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    log("warmup done")
-
-    stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        step()
-        ev[k][1].record(stream)
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t_start
-    kernel_ms = np.array([a.elapsed_time(b) for a, b in ev])
-    ms_per_step = t_total * 1e3 / args.steps
-    value = n / (ms_per_step / 1e3)
-    log("timed %d steps: %.3f ms/step, kernel avg %.3f ms" % (args.steps, ms_per_step,
-                                                               kernel_ms.mean()))
-
-    # algorithmic bytes per pair: 12 (n_A + n_B) FP32 hull bytes, 2 x 64 pose and
-    # parameters, 8 pair ids, 60 outputs (dist, two closest points, status)
-    bytes_per_launch = float(np.sum(w.hull_bytes_per_pair()) + n * (128 + 8 + 60))
-    achieved = bytes_per_launch / (kernel_ms.mean() / 1e3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "kernel": "gjk_distance_kernel",
-                "kernel_ms": round(float(kernel_ms.mean()), 4),
-                "bytes_per_launch": bytes_per_launch}
-
-    iters = out.iterations.cpu().numpy()
+    head = run_pipeline_config(w, args, lanes, args.steps, args.warmup,
+                               "mixed_pairs", parity=not args.no_parity, cpu=not args.no_cpu)
+    log("headline: %.3f ms/step" % head["ms_per_step"])
     result = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": head["value"],
         "unit": "pair-queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 simplex / f32 hull sweeps",
+        "dtype": "f64 (simplex, EPA) + f32 hull sweeps",
         "data": "synthetic",
-        "config": {"workload": "mixed_pairs (1,048,576 mixed-primitive GJK pairs)",
-                   "pairs": n, "seed": args.seed, "hull_vertices": "log-uniform 8..4096",
-                   "query": "GJK distance + closest points", "epa": False,
-                   "lanes_per_pair": lanes or list(__import__("distance3d_amd._native",
-                                                              fromlist=["x"]).DEFAULT_LANES),
-                   "mean_iterations": round(float(iters.mean()), 3)},
-        "roofline": roofline,
+        "config": {"workload": "mixed_pairs: 1,048,576 mixed-primitive pairs, GJK + EPA on "
+                               "intersecting pairs",
+                   "pairs": w.n_pairs, "seed": args.seed, "hull_vertices": "log-uniform 8..4096",
+                   "lanes_per_pair": lanes or list(_native.DEFAULT_LANES)},
+        "roofline": dict(head["roofline"], traffic=None,
+                         kernel="gjk_epa pipeline step (sort + 8 work-class GJK kernels + "
+                                "compaction + EPA)", kernel_ms=head["event_ms_per_step"]),
     }
-
-    if not args.no_parity or not args.no_cpu:
-        from oracle import oracle
-    if not args.no_parity:
-        t0 = time.time()
-        o = oracle.gjk_distance(w.scene, w.pair_a, w.pair_b)
-        dist = out.dist.cpu().numpy()
-        sep = (o["dist"] > 1e-9) & (o["dist"] < np.finfo(float).max)
-        rel = np.abs(dist[sep] - o["dist"][sep]) / o["dist"][sep]
-        band = (o["dist"] < 1e-9) | (dist < 1e-9)
-        flag_mismatch = int(np.sum(((dist == 0.0) != (o["dist"] == 0.0)) & ~band))
-        result["parity"] = {
-            "vs": "FP64 oracle", "pairs": n,
-            "max_rel_dist_err": float(rel.max(initial=0.0)),
-            "p99_rel_dist_err": float(np.percentile(rel, 99)) if len(rel) else 0.0,
-            "bit_identical_frac": float(np.mean(dist == o["dist"])),
-            "flag_mismatches_outside_1e-9_band": flag_mismatch,
-            "intersecting_frac": float(np.mean(o["dist"] == 0.0)),
-        }
-        log("parity checked in %.1fs" % (time.time() - t0))
-    if not args.no_cpu:
-        cores = oracle.num_threads()
-        oracle.gjk_distance(w.scene, w.pair_a[:65536], w.pair_b[:65536])  # warm-up
-        times = []
-        for _ in range(args.cpu_runs):
-            t0 = time.perf_counter()
-            oracle.gjk_distance(w.scene, w.pair_a, w.pair_b)
-            times.append(time.perf_counter() - t0)
-        cpu_rate = n / float(np.median(times))
-        result["cpu_baseline"] = {
-            "value": round(cpu_rate, 1), "unit": "pair-queries/s", "cores": cores,
-            "kind": "port",
-            "sample": "all %d pairs of the same workload, median of %d runs (%.1fs total), "
-                      "C FP64 restatement with OpenMP" % (n, args.cpu_runs, sum(times))}
-        result["speedup_vs_cpu"] = round(value / cpu_rate, 2)
+    if "parity" in head:
+        result["parity"] = head["parity"]
+    if "cpu_baseline" in head:
+        result["cpu_baseline"] = head["cpu_baseline"]
+        result["speedup_vs_cpu"] = round(head["value"] / head["cpu_baseline"]["value"], 2)
+    if not args.no_extra:
+        only = set(args.only.split(",")) if args.only else {"hull", "epa", "broad", "serving"}
+        cfgs = {}
+        if "hull" in only:
+            cfgs["hull_pairs"] = run_pipeline_config(workloads.hull_pairs(), args, lanes, 20, 3,
+                                                     "hull_pairs (4096 x 32-vertex hulls)",
+                                                     parity=not args.no_parity)
+        if "epa" in only:
+            t0 = time.time()
+            we = workloads.epa_pairs()
+            log("epa workload in %.1fs" % (time.time() - t0))
+            cfgs["epa_pairs"] = run_pipeline_config(we, args, lanes, 5, 1,
+                                                    "epa_pairs (262,144 overlapping hull pairs)",
+                                                    parity=not args.no_parity)
+        if "broad" in only:
+            cfgs["broad_phase"] = run_broad(args)
+        if "serving" in only:
+            cfgs["serving"] = run_serving(args, lanes)
+        result["configs"] = cfgs
     print(json.dumps(result), flush=True)
 
 

@@ -71,6 +71,9 @@ _SIGNATURES = {
                                   _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
     "d3d_gjk_intersection_batch": [_SCENE, _P, _P, _I64, _D, _I32, _P, _P, _P],
     "d3d_epa_batch": [_SCENE, _P, _P, _I64, _P, _P, _I32, _I32, _D, _P, _P, _P, _P],
+    "d3d_gjk_epa_batch_ws": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params), _I32, _I32,
+                             _D, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
+    "d3d_instance_pairs": [_SCENE, _P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P],
     "d3d_bvh_build": [_P, _I64, _P, _I64, _P],
     "d3d_bvh_query": [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P],
 }
@@ -83,7 +86,7 @@ _initialised = False
 def exported_symbols():
     """Names of the C-ABI functions this binding uses."""
     return (["d3d_abi_version", "d3d_last_error", "d3d_init", "d3d_gjk_workspace_bytes",
-             "d3d_bvh_bytes"] + list(_SIGNATURES))
+             "d3d_bvh_bytes", "d3d_gjk_epa_workspace_bytes"] + list(_SIGNATURES))
 
 
 def load_library(path=LIB_PATH):
@@ -100,7 +103,8 @@ def load_library(path=LIB_PATH):
             lib.d3d_last_error.restype = ctypes.c_char_p
             lib.d3d_init.argtypes = [ctypes.c_int]
             lib.d3d_init.restype = ctypes.c_int
-            for name in ("d3d_gjk_workspace_bytes", "d3d_bvh_bytes"):
+            for name in ("d3d_gjk_workspace_bytes", "d3d_bvh_bytes",
+                         "d3d_gjk_epa_workspace_bytes"):
                 getattr(lib, name).argtypes = [ctypes.c_int64]
                 getattr(lib, name).restype = ctypes.c_int64
             for name, argtypes in _SIGNATURES.items():

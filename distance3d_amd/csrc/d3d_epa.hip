@@ -462,6 +462,27 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
     }
 }
 
+int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n_max,
+               const int64_t *n_dev, const int32_t *work, const double *simplex,
+               const int32_t *npts, int max_iter, int max_faces, double eps, double *out,
+               int32_t *status, int32_t *iters, hipStream_t stream) {
+    if (n_max <= 0) return D3D_OK;
+    int64_t grid = n_max;
+    if (n_dev) {
+        int cus = 256, dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const int64_t cap = (int64_t)cus * 8;
+        if (grid > cap) grid = cap;
+    } else if (grid > 65536) {
+        grid = 65536;
+    }
+    hipLaunchKernelGGL(epa_kernel, dim3((unsigned)grid), dim3(kEpaBlock), 0, stream, s, pa, pb, n_max,
+                       n_dev, work, simplex, npts, max_iter, max_faces, eps, out, status, iters);
+    D3D_CHECK_LAUNCH("epa_kernel");
+    return D3D_OK;
+}
+
 }  // namespace d3d
 
 using namespace d3d;
@@ -478,11 +499,7 @@ extern "C" int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, cons
     D3D_CHECK_ARG(max_iterations >= 1 && max_iterations <= kEpaMaxIter,
                   "max_iterations must be in [1, 64]");
     D3D_CHECK_ARG(max_faces >= 4 && max_faces <= kEpaMaxFaces, "max_faces must be in [4, 128]");
-    const int64_t grid = n < 65536 ? n : 65536;
-    hipLaunchKernelGGL(epa_kernel, dim3((unsigned)grid), dim3(kEpaBlock), 0, (hipStream_t)stream,
-                       *scene, pair_a, pair_b, n, (const int64_t *)nullptr,
-                       (const int32_t *)nullptr, simplex, n_points, (int)max_iterations,
-                       (int)max_faces, epsilon, out, status, iterations);
-    D3D_CHECK_LAUNCH("epa_kernel");
-    return D3D_OK;
+    return launch_epa(*scene, pair_a, pair_b, n, nullptr, nullptr, simplex, n_points,
+                      (int)max_iterations, (int)max_faces, epsilon, out, status, iterations,
+                      (hipStream_t)stream);
 }

@@ -611,6 +611,84 @@ static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, 
     return D3D_OK;
 }
 
+// ------------------------------------------------- GJK -> EPA pipeline
+
+__global__ void zero_u64_kernel(unsigned long long *c) { *c = 0ull; }
+
+// Intersecting pairs (GJK status INTERSECT) are appended to `work` with one
+// atomic per wavefront; the others get EPA status -1 and zero outputs.
+__global__ __launch_bounds__(kQBlock) void epa_compact_kernel(const int32_t *__restrict__ status,
+                                                              int64_t n, int32_t *__restrict__ work,
+                                                              unsigned long long *__restrict__ count,
+                                                              double *__restrict__ epa_out,
+                                                              int32_t *__restrict__ epa_status) {
+    const int64_t i = (int64_t)blockIdx.x * kQBlock + threadIdx.x;
+    const bool valid = i < n;
+    const bool hit = valid && (status[i] & 0xFF) == D3D_INTERSECT;
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+        const int lane = lane_id();
+        const int first = __ffsll((long long)m) - 1;
+        unsigned long long base = 0;
+        if (lane == first) base = atomicAdd(count, (unsigned long long)__popcll(m));
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)base, first);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(base >> 32), first);
+        work[(((unsigned long long)hi << 32) | lo) + __popcll(m & lanemask_lt(lane))] = (int32_t)i;
+    } else if (valid) {
+#pragma unroll
+        for (int k = 0; k < 13; ++k) epa_out[13 * i + k] = 0.0;
+        epa_status[i] = -1;
+    }
+}
+
+// Per-pair collider instances: record 2i is scene record a[i] posed by
+// pose_a[i] (R row-major, t: 12 doubles), record 2i+1 is b[i] posed by
+// pose_b[i]; NULL poses keep the scene's.  Vertex offsets are copied, so the
+// instances share the scene's resident vertex pools.
+__global__ __launch_bounds__(kQBlock) void instance_kernel(
+    d3d_scene s, const int32_t *__restrict__ a, const int32_t *__restrict__ b,
+    const double *__restrict__ pose_a, const double *__restrict__ pose_b,
+    const int32_t *__restrict__ pidx_a, const int32_t *__restrict__ pidx_b, int64_t n_poses,
+    int64_t n, double *__restrict__ f64, int32_t *__restrict__ i32) {
+    const int64_t t = (int64_t)blockIdx.x * kQBlock + threadIdx.x;
+    if (t >= 2 * n) return;
+    const int64_t pair = t >> 1;
+    const int side = (int)(t & 1);
+    const int k = side ? b[pair] : a[pair];
+    const double *pose = side ? pose_b : pose_a;
+    const int32_t *pidx = side ? pidx_b : pidx_a;
+    int64_t prow = pair;
+    if (pidx) prow = pidx[pair];
+    double *dst = f64 + 16 * t;
+    int32_t *di = i32 + 4 * t;
+    if (k < 0 || (int64_t)k >= s.n_shapes || (pose && pidx && (prow < 0 || prow >= n_poses))) {
+        for (int j = 0; j < 16; ++j) dst[j] = 0.0;
+        di[0] = -1;  // rejected as bad input downstream
+        di[1] = di[2] = 0;
+        di[3] = -1;
+        return;
+    }
+    const double *src = s.shape_f64 + 16 * (int64_t)k;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) dst[j] = src[j];
+    if (pose) {
+#pragma unroll
+        for (int j = 0; j < 12; ++j) dst[j] = pose[12 * prow + j];
+    }
+    const int32_t *si = s.shape_i32 + 4 * (int64_t)k;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) di[j] = si[j];
+}
+
+__global__ __launch_bounds__(kQBlock) void iota_pairs_kernel(int32_t *__restrict__ a,
+                                                             int32_t *__restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kQBlock + threadIdx.x;
+    if (i < n) {
+        a[i] = (int32_t)(2 * i);
+        b[i] = (int32_t)(2 * i + 1);
+    }
+}
+
 }  // namespace d3d
 
 using namespace d3d;
@@ -677,4 +755,75 @@ extern "C" int d3d_gjk_distance_batch_ws(const d3d_scene *scene, const int32_t *
     D3D_CHECK_ARG(workspace != nullptr, "workspace is NULL");
     return gjk_distance_impl(scene, pair_a, pair_b, n, params, dist, cp_a, cp_b, status, iterations,
                              simplex, n_points, workspace, workspace_bytes, stream);
+}
+
+// Workspace of the fused pipeline: GJK queue | simplex (n*36 f64) |
+// n_points (n) | EPA work list (n) | count (u64)
+static inline int64_t a256(int64_t b) { return (b + 255) / 256 * 256; }
+
+extern "C" int64_t d3d_gjk_epa_workspace_bytes(int64_t n) {
+    if (n < 0) return -1;
+    return a256(d3d_gjk_workspace_bytes(n)) + a256(n * 36 * 8) + a256(n * 4) + a256(n * 4) + 256;
+}
+
+extern "C" int d3d_gjk_epa_batch_ws(const d3d_scene *scene, const int32_t *pair_a,
+                                    const int32_t *pair_b, int64_t n, const d3d_gjk_params *params,
+                                    int32_t epa_max_iterations, int32_t epa_max_faces,
+                                    double epa_epsilon, double *dist, double *cp_a, double *cp_b,
+                                    int32_t *status, int32_t *iterations, double *epa_out,
+                                    int32_t *epa_status, void *workspace, int64_t workspace_bytes,
+                                    void *stream) {
+    clear_error();
+    if (int rc = check_scene(scene)) return rc;
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(workspace && epa_out && epa_status, "NULL buffer");
+    D3D_CHECK_ARG(workspace_bytes >= d3d_gjk_epa_workspace_bytes(n), "workspace too small");
+    D3D_CHECK_ARG(epa_max_iterations >= 1 && epa_max_iterations <= 64,
+                  "epa_max_iterations must be in [1, 64]");
+    D3D_CHECK_ARG(epa_max_faces >= 4 && epa_max_faces <= 128, "epa_max_faces must be in [4, 128]");
+    char *w = (char *)workspace;
+    const int64_t gq = a256(d3d_gjk_workspace_bytes(n));
+    double *simplex = (double *)(w + gq);
+    int32_t *npts = (int32_t *)(w + gq + a256(n * 36 * 8));
+    int32_t *work = (int32_t *)(w + gq + a256(n * 36 * 8) + a256(n * 4));
+    unsigned long long *count =
+        (unsigned long long *)(w + gq + a256(n * 36 * 8) + a256(n * 4) + a256(n * 4));
+    if (int rc = gjk_distance_impl(scene, pair_a, pair_b, n, params, dist, cp_a, cp_b, status,
+                                   iterations, simplex, npts, workspace, gq, stream))
+        return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, st, count);
+    D3D_CHECK_LAUNCH("zero_u64_kernel");
+    hipLaunchKernelGGL(epa_compact_kernel, dim3(grid_for(n, kQBlock)), dim3(kQBlock), 0, st, status, n,
+                       work, count, epa_out, epa_status);
+    D3D_CHECK_LAUNCH("epa_compact_kernel");
+    return launch_epa(*scene, pair_a, pair_b, n, (const int64_t *)count, work, simplex, npts,
+                      (int)epa_max_iterations, (int)epa_max_faces, epa_epsilon, epa_out, epa_status,
+                      nullptr, st);
+}
+
+extern "C" int d3d_instance_pairs(const d3d_scene *scene, const int32_t *pair_a,
+                                  const int32_t *pair_b, const double *pose_a,
+                                  const double *pose_b, const int32_t *pose_idx_a,
+                                  const int32_t *pose_idx_b, int64_t n_poses, int64_t n,
+                                  double *inst_f64, int32_t *inst_i32, int32_t *inst_a,
+                                  int32_t *inst_b, void *stream) {
+    clear_error();
+    if (int rc = check_scene(scene)) return rc;
+    D3D_CHECK_ARG(n >= 0 && n < (int64_t)0x3fffffff, "n must be in [0, 2^30)");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(pair_a && pair_b && inst_f64 && inst_i32, "NULL buffer");
+    hipStream_t st = (hipStream_t)stream;
+    D3D_CHECK_ARG(!(pose_idx_a || pose_idx_b) || n_poses > 0, "pose indices need n_poses > 0");
+    hipLaunchKernelGGL(instance_kernel, dim3(grid_for(2 * n, kQBlock)), dim3(kQBlock), 0, st, *scene,
+                       pair_a, pair_b, pose_a, pose_b, pose_idx_a, pose_idx_b, n_poses, n, inst_f64,
+                       inst_i32);
+    D3D_CHECK_LAUNCH("instance_kernel");
+    if (inst_a && inst_b) {
+        hipLaunchKernelGGL(iota_pairs_kernel, dim3(grid_for(n, kQBlock)), dim3(kQBlock), 0, st, inst_a,
+                           inst_b, n);
+        D3D_CHECK_LAUNCH("iota_pairs_kernel");
+    }
+    return D3D_OK;
 }

@@ -62,4 +62,11 @@ inline int check_scene(const d3d_scene *s) {
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// EPA launcher shared by d3d_epa_batch and the fused GJK+EPA pipeline: with
+// n_dev/work set, a persistent grid takes pair indices work[0 .. *n_dev).
+int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n_max,
+               const int64_t *n_dev, const int32_t *work, const double *simplex,
+               const int32_t *npts, int max_iter, int max_faces, double eps, double *out,
+               int32_t *status, int32_t *iters, hipStream_t stream);
+
 }  // namespace d3d

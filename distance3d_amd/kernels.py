@@ -222,3 +222,96 @@ def bvh_query(bvh, queries, cap=None, counter=None, stream=None, sort=True):
         key = pairs[:, 0].to(torch.int64) * bvh.n + pairs[:, 1].to(torch.int64)
         pairs = pairs[torch.argsort(key)]
     return pairs
+
+
+class SceneView:
+    """A collider table (device tensors) over another scene's vertex pools;
+    used for per-pair pose instances.  Duck-types DeviceScene for the batch
+    functions above."""
+
+    def __init__(self, base, shape_f64, shape_i32, n_shapes):
+        self.device = base.device
+        self.base = base
+        self.shape_f64 = shape_f64
+        self.shape_i32 = shape_i32
+        self.n_shapes = n_shapes
+        self._c = _native.d3d_scene(
+            shape_f64.data_ptr(), shape_i32.data_ptr(), base.vx.data_ptr(), base.vy.data_ptr(),
+            base.vz.data_ptr(), base.v64.data_ptr(), n_shapes, base.n_vertices, base.n_vertices64)
+
+    @property
+    def c(self):
+        import ctypes
+        return ctypes.byref(self._c)
+
+
+def instance_pairs(scene, pair_a, pair_b, pose_a=None, pose_b=None, pose_idx_a=None,
+                   pose_idx_b=None, n_poses=0, out=None, stream=None):
+    """Posed pair instances of ``scene`` colliders (see d3d_instance_pairs).
+    Returns (SceneView over 2n instances, pair_a', pair_b') with the pair
+    lists 2i / 2i+1.  ``out`` may pass preallocated (f64 (2n,16), i32 (2n,4),
+    a (n), b (n)) tensors."""
+    import torch
+    n = int(pair_a.numel())
+    dev = scene.device
+    if out is None:
+        out = (torch.empty((2 * n, 16), dtype=torch.float64, device=dev),
+               torch.empty((2 * n, 4), dtype=torch.int32, device=dev),
+               torch.empty(n, dtype=torch.int32, device=dev),
+               torch.empty(n, dtype=torch.int32, device=dev))
+    f64, i32, ia, ib = out
+    check(lib().d3d_instance_pairs(scene.c, ptr(pair_a), ptr(pair_b), ptr(pose_a), ptr(pose_b),
+                                   ptr(pose_idx_a), ptr(pose_idx_b), int(n_poses), n, ptr(f64),
+                                   ptr(i32), ptr(ia), ptr(ib), stream_ptr(stream)),
+          "d3d_instance_pairs")
+    return SceneView(scene, f64, i32, 2 * n), ia[:n], ib[:n]
+
+
+def gjk_epa_workspace_bytes(n):
+    return int(_native.load_library().d3d_gjk_epa_workspace_bytes(int(n)))
+
+
+class GjkEpaResult:
+    """Outputs of :func:`gjk_epa_batch` (device tensors)."""
+
+    def __init__(self, dist, cp_a, cp_b, status, epa_out, epa_status):
+        self.dist = dist
+        self.cp_a = cp_a
+        self.cp_b = cp_b
+        self.status = status
+        self.epa_out = epa_out
+        self.epa_status = epa_status
+
+    @staticmethod
+    def empty(n, device):
+        import torch
+        return GjkEpaResult(torch.empty(n, dtype=torch.float64, device=device),
+                            torch.empty((n, 3), dtype=torch.float64, device=device),
+                            torch.empty((n, 3), dtype=torch.float64, device=device),
+                            torch.empty(n, dtype=torch.int32, device=device),
+                            torch.empty((n, 13), dtype=torch.float64, device=device),
+                            torch.empty(n, dtype=torch.int32, device=device))
+
+
+def gjk_epa_batch(scene, pair_a, pair_b, out=None, workspace=None, lanes_per_pair=None,
+                  epa_max_iterations=64, epa_max_faces=128, epa_epsilon=1e-8, stream=None,
+                  **gjk_kw):
+    """GJK distance for every pair and EPA for the intersecting ones, in one
+    device pipeline (no host synchronisation).  ``epa_status`` is -1 where
+    GJK found no intersection."""
+    import ctypes
+    import torch
+    a = _dev_i32(pair_a, scene.device)
+    b = _dev_i32(pair_b, scene.device)
+    n = a.numel()
+    if out is None:
+        out = GjkEpaResult.empty(n, scene.device)
+    if workspace is None:
+        workspace = torch.empty(gjk_epa_workspace_bytes(n), dtype=torch.uint8, device=scene.device)
+    prm = _native.gjk_params(lanes_per_pair=lanes_per_pair, **gjk_kw)
+    check(lib().d3d_gjk_epa_batch_ws(
+        scene.c, ptr(a), ptr(b), n, ctypes.byref(prm), int(epa_max_iterations),
+        int(epa_max_faces), float(epa_epsilon), ptr(out.dist), ptr(out.cp_a), ptr(out.cp_b),
+        ptr(out.status), None, ptr(out.epa_out), ptr(out.epa_status), ptr(workspace),
+        workspace.numel(), stream_ptr(stream)), "d3d_gjk_epa_batch_ws")
+    return out

@@ -17,6 +17,10 @@ FP64 pose is the geometry both the GPU and the FP64 checker see.
   0.01..0.5 per axis) and 65,536 capsules (r U(0.05, 0.5), h U(0.1, 2)) with
   centres uniform in [0, 100]^3, seed 3; boxes are shapes 0..n-1, capsules
   follow.
+* ``serving_workload`` -- a resident scene of 512 colliders (1/3 capsules, 1/3
+  boxes, 1/3 hulls with 64..1024 vertices), a table of random poses in a
+  [0, 4]^3 workspace, and a Poisson stream of requests (mean rate 20,000/s)
+  of log-uniform 1..4096 pairs, each pair with its own pose rows, seed 4.
 """
 import numpy as np
 
@@ -215,3 +219,65 @@ def broad_phase_scene(seed=3, n_boxes=1 << 20, n_capsules=65536, extent=100.0):
     return Workload("broad_phase_scene", scene, np.zeros(0, dtype=np.int32),
                     np.zeros(0, dtype=np.int32),
                     {"seed": seed, "n_boxes": n_boxes, "n_capsules": n_capsules})
+
+
+def serving_workload(seed=4, n_colliders=512, n_requests=50000, rate=20000.0, max_pairs=4096,
+                     n_poses=65536, workspace=4.0):
+    """(HostScene, poses (P,12), list of serving.Request)."""
+    from .serving import Request
+    rng = np.random.default_rng(seed)
+    kind = np.arange(n_colliders) % 3
+    types = np.where(kind == 0, _native.CAPSULE, np.where(kind == 1, _native.BOX, _native.HULL))
+    types = types.astype(np.int32)
+    R = uniform_rotations(rng, n_colliders)
+    t = rng.uniform(0.0, workspace, (n_colliders, 3))
+    params = np.zeros((n_colliders, 3))
+    cap = types == _native.CAPSULE
+    params[cap, 0] = rng.uniform(0.05, 0.3, int(cap.sum()))
+    params[cap, 1] = rng.uniform(0.2, 1.0, int(cap.sum()))
+    box = types == _native.BOX
+    params[box] = rng.uniform(0.1, 1.0, (int(box.sum()), 3))
+    hull = types == _native.HULL
+    counts = np.zeros(n_colliders, dtype=np.int64)
+    counts[hull] = rng.integers(64, 1025, int(hull.sum()))
+    total = int(counts.sum())
+    owner_scale = np.repeat(rng.uniform(0.2, 0.6, int(hull.sum())), counts[hull])
+    v32 = (_sphere_points(rng, total) * (owner_scale * (1.0 + rng.uniform(-0.1, 0.1, total)))[:, None]
+           ).astype(np.float32)
+    scene = _scene_from(types, R, t, params, counts, v32)
+    PR = uniform_rotations(rng, n_poses)
+    poses = np.concatenate([PR.reshape(n_poses, 9), rng.uniform(0.0, workspace, (n_poses, 3))],
+                           axis=1)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    arrival = np.cumsum(gaps) - gaps[0]
+    sizes = np.clip(np.rint(np.exp(rng.uniform(0.0, np.log(max_pairs), n_requests))), 1,
+                    max_pairs).astype(np.int64)
+    total_pairs = int(sizes.sum())
+    a = rng.integers(0, n_colliders, total_pairs)
+    b = (a + rng.integers(1, n_colliders, total_pairs)) % n_colliders
+    pa = rng.integers(0, n_poses, total_pairs)
+    pb = rng.integers(0, n_poses, total_pairs)
+    reqs = []
+    off = 0
+    for k in range(n_requests):
+        e = off + int(sizes[k])
+        reqs.append(Request(k, arrival[k], a[off:e], b[off:e], pa[off:e], pb[off:e]))
+        off = e
+    return scene, poses, reqs
+
+
+def posed_instances(scene, poses, pair_a, pair_b, pose_a, pose_b):
+    """Host-side instance table of posed pairs: rows 2i / 2i+1 are scene rows
+    pair_a[i] / pair_b[i] with pose rows pose_a[i] / pose_b[i] (the checker's
+    view of d3d_instance_pairs)."""
+    n = len(pair_a)
+    f = np.empty((2 * n, F64_COLS))
+    f[0::2] = scene.shape_f64[pair_a]
+    f[1::2] = scene.shape_f64[pair_b]
+    P = np.asarray(poses, dtype=np.float64).reshape(-1, 12)
+    f[0::2, :12] = P[pose_a]
+    f[1::2, :12] = P[pose_b]
+    i = np.empty((2 * n, I32_COLS), dtype=np.int32)
+    i[0::2] = scene.shape_i32[pair_a]
+    i[1::2] = scene.shape_i32[pair_b]
+    return HostScene(f, i, scene.vx, scene.vy, scene.vz, scene.v64)

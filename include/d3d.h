@@ -190,6 +190,40 @@ int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *
                   int32_t *status, int32_t *iterations, void *stream);
 
 /*
+ * GJK then EPA in one device-side pipeline (gjk.gjk followed by epa.epa on
+ * the intersecting pairs, gjk/_gjk_jolt.py:138-221 and epa.py:9-78): GJK as
+ * d3d_gjk_distance_batch, intersecting pairs compacted on the device, EPA on
+ * them by a persistent kernel; no host synchronisation.
+ *   epa_out (n,13) f64 as d3d_epa_batch; epa_status (n) int32 as
+ *   d3d_epa_batch, or -1 for pairs that do not intersect (zero outputs).
+ *   workspace: caller-owned, >= d3d_gjk_epa_workspace_bytes(n) bytes.
+ */
+int64_t d3d_gjk_epa_workspace_bytes(int64_t n);
+int d3d_gjk_epa_batch_ws(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
+                         int64_t n, const d3d_gjk_params *params, int32_t epa_max_iterations,
+                         int32_t epa_max_faces, double epa_epsilon, double *dist, double *cp_a,
+                         double *cp_b, int32_t *status, int32_t *iterations, double *epa_out,
+                         int32_t *epa_status, void *workspace, int64_t workspace_bytes,
+                         void *stream);
+
+/*
+ * Pose-updated pair instances (collider.update_pose per pair,
+ * colliders.py:174-178, 276-279, 330-333, ...): writes a collider table of
+ * 2n records -- record 2i = scene collider pair_a[i] with pose row
+ * pose_a[i], record 2i+1 = pair_b[i] with pose row pose_b[i] (12 f64 per
+ * row: R row-major, t; NULL keeps the scene pose) -- into inst_f64 (2n,16) /
+ * inst_i32 (2n,4), and, when inst_a/inst_b are given, the pair lists 2i /
+ * 2i+1.  With pose_idx_a / pose_idx_b (n int32) the rows are
+ * pose_a[pose_idx_a[i]] of a table of n_poses rows (a pose table kept on the
+ * device).  The instances keep the scene's vertex offsets: a d3d_scene made
+ * of this table and the scene's vertex pools evaluates the posed pairs.
+ */
+int d3d_instance_pairs(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
+                       const double *pose_a, const double *pose_b, const int32_t *pose_idx_a,
+                       const int32_t *pose_idx_b, int64_t n_poses, int64_t n, double *inst_f64,
+                       int32_t *inst_i32, int32_t *inst_a, int32_t *inst_b, void *stream);
+
+/*
  * Broad phase: linear BVH over n AABBs and overlap queries
  * (aabb_tree.py:15-191 AabbTree, 381-403 query_overlap, 465-527
  * all_aabbs_overlap / aabb_overlap).

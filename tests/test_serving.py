@@ -1,0 +1,111 @@
+"""Serving loop: admission / batching logic on the CPU with a stand-in slot,
+and on the GPU the alone-vs-in-the-mix equality and oracle parity of posed
+pairs."""
+import numpy as np
+import pytest
+
+from distance3d_amd import serving, workloads
+from oracle import oracle
+
+
+class _FakeSlot:
+    """Stand-in slot: outputs are a deterministic function of each pair and
+    complete after ``delay`` polls; records what it was loaded with."""
+
+    log = []
+
+    def __init__(self, server, capacity, delay=2):
+        self.server = server
+        self.capacity = capacity
+        self.delay = delay
+        self.batch = []
+
+    def prepare(self):
+        pass
+
+    def load(self, requests):
+        assert sum(r.n_pairs for r in requests) <= self.capacity
+        self.batch = list(requests)
+        _FakeSlot.log.append([r.rid for r in requests])
+
+    def launch(self):
+        self.polls = 0
+
+    def done(self):
+        self.polls += 1
+        return self.polls > self.delay
+
+    def results(self, now):
+        out = []
+        for r in self.batch:
+            k = r.n_pairs
+            d = (r.pair_a * 1000 + r.pair_b).astype(float)
+            out.append(serving.Result(r.rid, d, np.zeros((k, 3)), np.zeros((k, 3)),
+                                      np.zeros(k, np.int32), np.zeros((k, 13)),
+                                      np.zeros(k, np.int32), now - r.arrival))
+        return out
+
+
+class _FakeServer(serving.CollisionServer):
+    def __init__(self, n_slots, slot_pairs):
+        self.slots = [_FakeSlot(self, slot_pairs) for _ in range(n_slots)]
+
+
+def _reqs(rs, n, max_pairs):
+    out = []
+    for k in range(n):
+        m = rs.randint(1, max_pairs + 1)
+        out.append(serving.Request(k, 0.0, rs.randint(0, 50, m), rs.randint(0, 50, m),
+                                   np.zeros(m), np.zeros(m)))
+    return out
+
+
+def test_admission_fifo_capacity_and_completeness():
+    rs = np.random.RandomState(0)
+    reqs = _reqs(rs, 200, 300)
+    _FakeSlot.log = []
+    srv = _FakeServer(n_slots=3, slot_pairs=700)
+    res, stats = srv.serve(reqs, realtime=False)
+    assert sorted(res) == list(range(200))
+    order = [rid for batch in _FakeSlot.log for rid in batch]
+    assert order == sorted(order)  # arrival (FIFO) order, nothing skipped
+    assert stats["batches"] == len(_FakeSlot.log)
+    for r in reqs:
+        assert np.array_equal(res[r.rid].dist, (r.pair_a * 1000 + r.pair_b).astype(float))
+
+
+def test_oversized_request_rejected():
+    srv = _FakeServer(n_slots=1, slot_pairs=10)
+    big = serving.Request(0, 0.0, np.zeros(11), np.zeros(11), np.zeros(11), np.zeros(11))
+    with pytest.raises(ValueError):
+        srv.serve([big], realtime=False)
+
+
+@pytest.mark.gpu
+def test_alone_vs_mix_and_oracle(gpu):
+    scene, poses, reqs = workloads.serving_workload(seed=11, n_colliders=96, n_requests=120,
+                                                    max_pairs=512)
+    dev = scene.to_device()
+    srv = serving.CollisionServer(dev, poses, n_slots=2, slot_pairs=4096)
+    mix, stats = srv.serve(reqs, realtime=False)
+    assert stats["requests"] == len(reqs)
+    for r in reqs[:12]:
+        alone, _ = srv.serve([serving.Request(r.rid, 0.0, r.pair_a, r.pair_b, r.pose_a,
+                                              r.pose_b)], realtime=False)
+        assert alone[r.rid].same_as(mix[r.rid])
+    # oracle parity of the posed pairs (GJK, then EPA on intersecting pairs)
+    for r in reqs[:40]:
+        host = workloads.posed_instances(scene, poses, r.pair_a, r.pair_b, r.pose_a, r.pose_b)
+        k = r.n_pairs
+        a, b = np.arange(0, 2 * k, 2), np.arange(1, 2 * k, 2)
+        g = oracle.gjk_distance(host, a, b, want_simplex=True)
+        out = mix[r.rid]
+        assert np.array_equal(out.dist == 0.0, g["dist"] == 0.0)
+        sep = g["dist"] > 1e-9
+        assert np.all(np.abs(out.dist[sep] - g["dist"][sep]) <= 1e-5 * g["dist"][sep])
+        hit = np.nonzero(g["dist"] == 0.0)[0]
+        if len(hit):
+            e = oracle.epa(host, a[hit], b[hit], g["simplex"][hit], g["n_points"][hit])
+            assert np.array_equal(out.epa_status[hit], e["status"])
+            assert np.all(np.abs(out.depth[hit] - e["depth"]) <= 1e-6 * np.maximum(1, e["depth"]))
+        assert np.all(out.epa_status[g["dist"] > 0.0] == -1)
