@@ -275,7 +275,8 @@ def main():
     import torch  # noqa: F401
     from distance3d_amd import _native, workloads
 
-    lanes = [int(x) for x in args.lanes.split(",")] if args.lanes else None
+    from distance3d_amd import autotune
+    lanes = [int(x) for x in args.lanes.split(",")] if args.lanes else autotune.tuned_lanes()
     t0 = time.time()
     w = workloads.mixed_pairs(seed=args.seed, n_pairs=args.pairs)
     log("workload generated in %.1fs: %d pairs, %d hull vertices"
