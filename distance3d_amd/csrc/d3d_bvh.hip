@@ -16,6 +16,7 @@
 // reference's closed-interval FP64 test (aabb_tree.py:503-527), so the pair
 // set equals the brute-force set of all_aabbs_overlap exactly.  Hits are
 // appended with one atomic per wavefront (ballot compaction).
+#include "d3d_bvh_core.hpp"
 #include "d3d_device.hpp"
 #include "d3d_runtime.hpp"
 
@@ -222,32 +223,60 @@ __global__ __launch_bounds__(1024) void rs_scan_kernel(int32_t *__restrict__ his
     }
 }
 
+// Stable scatter: items of a tile are ranked in index order.  Per chunk of
+// 256 items, each wavefront finds the lanes holding the same digit by
+// matching the 8 digit bits with ballots; a lane's position is the tile's
+// running count for its digit, plus the counts of the same digit in earlier
+// wavefronts of the chunk, plus its rank among the earlier lanes of its own
+// wavefront.  (LSD radix sort needs every pass to be stable.)
 __global__ __launch_bounds__(kBvhBlock) void rs_scatter_kernel(
     const uint32_t *__restrict__ keys, const int32_t *__restrict__ vals, int64_t n, int shift,
     const int32_t *__restrict__ offs, uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out) {
-    __shared__ int32_t h[256];
-    h[threadIdx.x] = offs[(int64_t)threadIdx.x * gridDim.x + blockIdx.x];
+    constexpr int kWaves = kBvhBlock / kWave;
+    __shared__ int32_t run[256];
+    __shared__ int32_t wcnt[kWaves][256];
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wave = t / kWave;
+    run[t] = offs[(int64_t)t * gridDim.x + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
     __syncthreads();
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
     for (int j = 0; j < kSortItems; ++j) {
-        const int64_t i = base + (int64_t)j * kBvhBlock + threadIdx.x;
-        if (i < n) {
-            const uint32_t k = keys[i];
-            const int pos = atomicAdd(&h[(k >> shift) & 255u], 1);
+        const int64_t i = base + (int64_t)j * kBvhBlock + t;
+        const bool valid = i < n;
+        const uint32_t k = valid ? keys[i] : 0u;
+        const int dig = (int)((k >> shift) & 255u);
+        unsigned long long same = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long m = __ballot((dig >> b) & 1);
+            same &= ((dig >> b) & 1) ? m : ~m;
+        }
+        const int rank = __popcll(same & lt);
+        if (valid && rank == 0) wcnt[wave][dig] = __popcll(same);
+        __syncthreads();
+        if (valid) {
+            int pos = run[dig] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][dig];
             keys_out[pos] = k;
             vals_out[pos] = vals[i];
         }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            add += wcnt[w][t];
+            wcnt[w][t] = 0;
+        }
+        run[t] += add;
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------- hierarchy
-
-__device__ __forceinline__ int delta(const uint32_t *__restrict__ k, int64_t n, int i, int j) {
-    if (j < 0 || (int64_t)j >= n) return -1;
-    const uint32_t a = k[i], b = k[j];
-    if (a == b) return 32 + __clz((unsigned)(i ^ j));
-    return __clz(a ^ b);
-}
 
 __global__ __launch_bounds__(kBvhBlock) void bvh_hierarchy_kernel(const uint32_t *__restrict__ keys,
                                                                   int64_t n,
@@ -258,25 +287,8 @@ __global__ __launch_bounds__(kBvhBlock) void bvh_hierarchy_kernel(const uint32_t
     const int64_t ii = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
     if (ii >= n - 1) return;
     const int i = (int)ii;
-    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = delta(keys, n, i, i - d);
-    int lmax = 2;
-    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-    const int j = i + l * d;
-    const int dnode = delta(keys, n, i, j);
-    int s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    const int gamma = i + s * d + (d < 0 ? -1 : 0);
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    const int leaf0 = (int)(n - 1);
-    const int lc = (lo == gamma) ? leaf0 + gamma : gamma;
-    const int rc = (hi == gamma + 1) ? leaf0 + gamma + 1 : gamma + 1;
+    int lc, rc;
+    karras_children(keys, n, i, &lc, &rc);
     nodes[i].left = lc;
     right[i] = rc;
     parent[lc] = i;
@@ -318,7 +330,8 @@ __global__ __launch_bounds__(kBvhBlock) void bvh_refit_kernel(const double *__re
     if (n == 1) return;
     __threadfence();
     int p = parent[leaf];
-    while (p >= 0) {
+    // at most n-1 internal ancestors; the cap only guards a malformed tree
+    for (int64_t steps = 0; p >= 0 && steps < n; ++steps) {
         if (atomicAdd(&flags[p], 1) == 0) return;
         __threadfence();
         const int a = nodes[p].left, b = right[p];
@@ -337,19 +350,18 @@ __global__ __launch_bounds__(kBvhBlock) void bvh_refit_kernel(const double *__re
 
 __global__ __launch_bounds__(kBvhBlock) void bvh_rope_kernel(int64_t n, BvhNode *__restrict__ nodes,
                                                              const int32_t *__restrict__ parent,
+                                                             const int32_t *__restrict__ left,
                                                              const int32_t *__restrict__ right) {
     const int64_t k = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
     if (k >= 2 * n - 1) return;
-    int cur = (int)k, esc = -1;
-    while (cur != 0) {
-        const int p = parent[cur];
-        if (nodes[p].left == cur) {
-            esc = right[p];
-            break;
-        }
-        cur = p;
-    }
-    nodes[k].escape = esc;
+    nodes[k].escape = rope_of((int)k, n, parent, left, right);
+}
+
+// internal nodes' left children as a plain array for rope_of
+__global__ __launch_bounds__(kBvhBlock) void bvh_left_kernel(int64_t n, const BvhNode *__restrict__ nodes,
+                                                             int32_t *__restrict__ left) {
+    const int64_t k = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
+    if (k < n - 1) left[k] = nodes[k].left;
 }
 
 // ------------------------------------------------------------- query
@@ -461,16 +473,16 @@ extern "C" int d3d_bvh_build(const double *aabbs, int64_t n, void *bvh, int64_t 
     hipLaunchKernelGGL(bvh_refit_kernel, dim3(nb), dim3(kBvhBlock), 0, st, aabbs, n, vin, v.leaf_orig,
                        v.nodes, v.parent, v.right, v.flags);
     D3D_CHECK_LAUNCH("bvh_refit_kernel");
+    // ropes; the refit flags (no longer needed) hold the left-child array
     if (n > 1) {
-        const int nn = (int)((2 * n - 1 + kBvhBlock - 1) / kBvhBlock);
-        hipLaunchKernelGGL(bvh_rope_kernel, dim3(nn), dim3(kBvhBlock), 0, st, n, v.nodes, v.parent,
-                           v.right);
-        D3D_CHECK_LAUNCH("bvh_rope_kernel");
-    } else {
-        hipLaunchKernelGGL(bvh_rope_kernel, dim3(1), dim3(kBvhBlock), 0, st, n, v.nodes, v.parent,
-                           v.right);
-        D3D_CHECK_LAUNCH("bvh_rope_kernel");
+        hipLaunchKernelGGL(bvh_left_kernel, dim3((unsigned)((n - 1 + kBvhBlock - 1) / kBvhBlock)),
+                           dim3(kBvhBlock), 0, st, n, v.nodes, v.flags);
+        D3D_CHECK_LAUNCH("bvh_left_kernel");
     }
+    const int nn = (int)((2 * n - 1 + kBvhBlock - 1) / kBvhBlock);
+    hipLaunchKernelGGL(bvh_rope_kernel, dim3(nn), dim3(kBvhBlock), 0, st, n, v.nodes, v.parent,
+                       v.flags, v.right);
+    D3D_CHECK_LAUNCH("bvh_rope_kernel");
     return D3D_OK;
 }
 

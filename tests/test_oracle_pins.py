@@ -514,3 +514,42 @@ def test_epa_seed_from_short_simplex():
     e = oracle.epa(s, [0], [1], g["simplex"], g["n_points"])
     assert e["status"][0] == 1
     assert e["depth"][0] == approx(1.0)
+
+
+def test_ellipse_aabb_equals_support_extents():  # test_geometry.py:70-86
+    rs = np.random.RandomState(873)
+    for _ in range(20):
+        T = gb.random_pose(rs)
+        radii = rs.uniform(0.1, 1.0, 2)
+        b = SceneBuilder()
+        b.add_ellipse(T[:3, 3], T[:3, :2].T, radii)
+        s = b.build()
+        box = oracle.aabbs(s, [0])[0]
+        lo = np.min(oracle.support(s, [0, 0, 0], -np.eye(3)), axis=0)
+        hi = np.max(oracle.support(s, [0, 0, 0], np.eye(3)), axis=0)
+        assert_array_almost_equal(box[:, 0], lo)
+        assert_array_almost_equal(box[:, 1], hi)
+
+
+def test_ellipsoid_aabb_axis_permutations():
+    """containment.ellipsoid_aabb reduces to the radii when the rotation
+    permutes the axes (closed form); a transposed or mis-indexed term moves
+    the radii to the wrong axes."""
+    import itertools
+    radii = np.array([0.3, 0.7, 1.1])
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product([-1, 1], repeat=3):
+            R = np.zeros((3, 3))
+            for i, j in enumerate(perm):
+                R[i, j] = signs[i]
+            if np.linalg.det(R) < 0:
+                continue
+            T = np.eye(4)
+            T[:3, :3] = R
+            T[:3, 3] = [1.0, -2.0, 0.5]
+            b = SceneBuilder()
+            b.add_ellipsoid(T, radii)
+            box = oracle.aabbs(b.build(), [0])[0]
+            ext = np.array([radii[perm[i]] for i in range(3)])
+            assert_array_almost_equal(box[:, 0], T[:3, 3] - ext)
+            assert_array_almost_equal(box[:, 1], T[:3, 3] + ext)
