@@ -46,6 +46,9 @@ def parse():
                    help="comma list of extra configs: hull,epa,broad,serving")
     p.add_argument("--cpu-runs", type=int, default=3)
     p.add_argument("--serving-requests", type=int, default=50000)
+    p.add_argument("--copy-calibration-mib", type=int, default=0,
+                   help="before the workload, run one device-to-device copy of this many MiB "
+                        "(a kernel of known traffic for scaling profiler byte counters)")
     return p.parse_args()
 
 
@@ -277,6 +280,14 @@ def main():
 
     from distance3d_amd import autotune
     lanes = [int(x) for x in args.lanes.split(",")] if args.lanes else autotune.tuned_lanes()
+    if args.copy_calibration_mib > 0:
+        src = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: %d MiB read + %d MiB written" % (args.copy_calibration_mib,
+                                                               args.copy_calibration_mib))
+        del src, dst
     t0 = time.time()
     w = workloads.mixed_pairs(seed=args.seed, n_pairs=args.pairs)
     log("workload generated in %.1fs: %d pairs, %d hull vertices"

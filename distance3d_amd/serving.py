@@ -90,8 +90,13 @@ class GpuSlot:
         dev = server.scene.device
         self.stream = torch.cuda.Stream(device=dev)
         self.event = torch.cuda.Event()
-        self.d_in = torch.empty((4, capacity), dtype=torch.int32, device=dev)
-        self.h_in = torch.empty((4, capacity), dtype=torch.int32, pin_memory=True)
+        # inputs of a batch padded to bucket b sit contiguously in the first
+        # 4b entries as [pair_a | pair_b | pose_a | pose_b], so the captured
+        # copy-in is one contiguous pinned -> device range (a strided host
+        # slice would be staged through a temporary that a graph captures
+        # once and never refreshes)
+        self.d_in = torch.empty(4 * capacity, dtype=torch.int32, device=dev)
+        self.h_in = torch.empty(4 * capacity, dtype=torch.int32, pin_memory=True)
         self.inst = (torch.empty((2 * capacity, 16), dtype=torch.float64, device=dev),
                      torch.empty((2 * capacity, 4), dtype=torch.int32, device=dev),
                      torch.empty(capacity, dtype=torch.int32, device=dev),
@@ -111,11 +116,12 @@ class GpuSlot:
 
     def _pipeline(self, n):
         srv = self.server
-        self.d_in[:, :n].copy_(self.h_in[:, :n], non_blocking=True)
+        self.d_in[:4 * n].copy_(self.h_in[:4 * n], non_blocking=True)
+        d = self.d_in
         f64, i32, ia, ib = self.inst
         view, pa, pb = kernels.instance_pairs(
-            srv.scene, self.d_in[0, :n], self.d_in[1, :n], srv.poses, srv.poses,
-            self.d_in[2, :n], self.d_in[3, :n], srv.n_poses,
+            srv.scene, d[0:n], d[n:2 * n], srv.poses, srv.poses,
+            d[2 * n:3 * n], d[3 * n:4 * n], srv.n_poses,
             out=(f64[:2 * n], i32[:2 * n], ia[:n], ib[:n]))
         o = self.out
         res = kernels.GjkEpaResult(o.dist[:n], o.cp_a[:n], o.cp_b[:n], o.status[:n],
@@ -135,7 +141,7 @@ class GpuSlot:
         for b in BUCKETS:
             if b > self.capacity:
                 break
-            self._pad(0, b)
+            self._pad(b, 0)
             with torch.cuda.stream(self.stream):
                 self._pipeline(b)
             self.stream.synchronize()
@@ -146,31 +152,31 @@ class GpuSlot:
                 self.graphs[b] = g
         self.stream.synchronize()
 
-    def _pad(self, start, end):
-        if end > start:
+    def _pad(self, bucket, start):
+        """Fill entries [start, bucket) of the bucket layout with the dummy pair."""
+        if bucket > start:
             h = self.h_in.numpy()
-            dummy = self.server.dummy
-            h[0, start:end] = dummy[0]
-            h[1, start:end] = dummy[1]
-            h[2, start:end] = dummy[2]
-            h[3, start:end] = dummy[3]
+            for c, v in enumerate(self.server.dummy):
+                h[c * bucket + start:(c + 1) * bucket] = v
 
     def load(self, requests):
+        total = sum(r.n_pairs for r in requests)
+        self.bucket = min(next((b for b in BUCKETS if b >= total), self.capacity),
+                          self.capacity)
+        b = self.bucket
         h = self.h_in.numpy()
         off = 0
         self.batch = []
         for r in requests:
             k = r.n_pairs
-            h[0, off:off + k] = r.pair_a
-            h[1, off:off + k] = r.pair_b
-            h[2, off:off + k] = r.pose_a
-            h[3, off:off + k] = r.pose_b
+            h[off:off + k] = r.pair_a
+            h[b + off:b + off + k] = r.pair_b
+            h[2 * b + off:2 * b + off + k] = r.pose_a
+            h[3 * b + off:3 * b + off + k] = r.pose_b
             self.batch.append((r, off, k))
             off += k
         self.n_used = off
-        self.bucket = next((b for b in BUCKETS if b >= off), self.capacity)
-        self.bucket = min(self.bucket, self.capacity)
-        self._pad(off, self.bucket)
+        self._pad(b, off)
 
     def launch(self):
         import torch

@@ -109,3 +109,26 @@ def test_alone_vs_mix_and_oracle(gpu):
             assert np.array_equal(out.epa_status[hit], e["status"])
             assert np.all(np.abs(out.depth[hit] - e["depth"]) <= 1e-6 * np.maximum(1, e["depth"]))
         assert np.all(out.epa_status[g["dist"] > 0.0] == -1)
+
+
+def test_slot_input_layout_is_contiguous_per_bucket():
+    """A loaded batch occupies the first 4*bucket staging entries as
+    [pair_a | pair_b | pose_a | pose_b] with dummy padding, so one contiguous
+    copy of that range carries the whole batch."""
+    import torch
+    slot = object.__new__(serving.GpuSlot)
+    slot.capacity = 4096
+    slot.server = type("S", (), {"dummy": (7, 8, 9, 10)})()
+    slot.h_in = torch.full((4 * 4096,), -1, dtype=torch.int32)
+    r1 = serving.Request(0, 0.0, [1, 2, 3], [4, 5, 6], [11, 12, 13], [14, 15, 16])
+    r2 = serving.Request(1, 0.0, [20], [21], [22], [23])
+    slot.load([r1, r2])
+    assert slot.bucket == 64 and slot.batch[1][1:] == (3, 1)
+    h = slot.h_in.numpy()[:4 * 64].reshape(4, 64)
+    assert list(h[:, :4].ravel()) == [1, 2, 3, 20, 4, 5, 6, 21, 11, 12, 13, 22, 14, 15, 16, 23]
+    assert np.array_equal(h[:, 4:], np.array([7, 8, 9, 10])[:, None].repeat(60, 1))
+    big = serving.Request(2, 0.0, np.arange(300), np.arange(300), np.zeros(300), np.zeros(300))
+    slot.load([big])
+    assert slot.bucket == 1024
+    h = slot.h_in.numpy()[:4 * 1024].reshape(4, 1024)
+    assert np.array_equal(h[0, :300], np.arange(300)) and np.all(h[1, 300:] == 8)
