@@ -41,7 +41,14 @@ class d3d_scene(ctypes.Structure):
         ("n_shapes", ctypes.c_int64),
         ("n_vertices", ctypes.c_int64),
         ("n_vertices64", ctypes.c_int64),
+        ("vorig", ctypes.c_void_p),
+        ("clusters", ctypes.c_void_p),
+        ("n_clusters", ctypes.c_int64),
     ]
+
+
+ABI_VERSION = 2
+CLUSTER_SIZE = 16
 
 
 class d3d_gjk_params(ctypes.Structure):
@@ -76,6 +83,9 @@ _SIGNATURES = {
     "d3d_instance_pairs": [_SCENE, _P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P],
     "d3d_bvh_build": [_P, _I64, _P, _I64, _P],
     "d3d_bvh_query": [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P],
+    "d3d_cluster_plan": [_P, _I64, _I32, _P, _P, _P],
+    "d3d_cluster_scene": [_P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I32, _I32,
+                          _P, _P, _P, _P, _P, _P, _P, _P],
 }
 
 _lock = threading.Lock()
@@ -100,6 +110,9 @@ def load_library(path=LIB_PATH):
                     "`make -C distance3d_amd/csrc`" % path)
             lib = ctypes.CDLL(path)
             lib.d3d_abi_version.restype = ctypes.c_int
+            if lib.d3d_abi_version() != ABI_VERSION:
+                raise D3DError("%s has ABI version %d, this binding needs %d: rebuild it"
+                               % (path, lib.d3d_abi_version(), ABI_VERSION))
             lib.d3d_last_error.restype = ctypes.c_char_p
             lib.d3d_init.argtypes = [ctypes.c_int]
             lib.d3d_init.restype = ctypes.c_int

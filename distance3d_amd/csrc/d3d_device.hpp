@@ -7,7 +7,7 @@
 // termination tests or the closest-point reconstruction is FP64, evaluated
 // left to right with one rounding per operation (the library is compiled with
 // -ffp-contract=off).  Only the hull argmax scan runs in FP32, and its winner is
-// certified against the FP64 argmax (see coop_hull_argmax).
+// certified against the FP64 argmax (see hull_argmax).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -143,7 +143,14 @@ __device__ __forceinline__ bool load_checked(const d3d_scene &s, int k, Shape &h
         const bool ok32 = h.nv >= 1 && h.voff >= 0 && (int64_t)h.voff + h.nv <= s.n_vertices;
         const bool ok64 =
             h.voff64 < 0 || ((int64_t)h.voff64 + h.nv <= s.n_vertices64 && s.v64 != nullptr);
-        if (!ok32 || !ok64) {
+        // cluster fields: p1 = first record, p2 = count (0 = unclustered)
+        const double c0 = h.p[1], cn = h.p[2];
+        const bool okc =
+            (cn == 0.0) ||
+            (s.clusters != nullptr && c0 >= 0.0 && c0 == floor(c0) &&
+             cn == (double)((h.nv + D3D_CLUSTER_SIZE - 1) / D3D_CLUSTER_SIZE) &&
+             c0 + cn <= (double)s.n_clusters);
+        if (!ok32 || !ok64 || !okc) {
             h = empty_shape();
             return false;
         }
@@ -174,61 +181,179 @@ __device__ __forceinline__ v3 hull_vertex(const d3d_scene &s, int voff, int voff
     return mk((double)s.vx[j], (double)s.vy[j], (double)s.vz[j]);
 }
 
-// Lane-serial FP64 argmax over one hull: vertices[argmax(vertices . l)] with
-// the first index on ties (colliders.py:131-132, np.argmax).
-__device__ __forceinline__ int serial_hull_argmax(const d3d_scene &s, int voff, int voff64, int nv,
-                                                  v3 l) {
-    double best = -INFINITY;
-    int bi = 0;
-    for (int k = 0; k < nv; ++k) {
-        v3 v = hull_vertex(s, voff, voff64, k);
-        double val = v.x * l.x + v.y * l.y + v.z * l.z;
-        if (val > best) {
-            best = val;
-            bi = k;
-        }
-    }
-    return bi;
+// A hull's vertex range and sweep metadata (see d3d.h for the clustered
+// layout: p1 = first cluster record, p2 = number of clusters, 0 = none).
+struct HullRef {
+    int voff, voff64, nv, coff, nc;
+    double L;
+};
+
+__device__ __forceinline__ HullRef hull_ref(const Shape &h) {
+    return HullRef{h.voff, h.voff64, h.nv, (int)h.p[1], (int)h.p[2], h.p[0]};
 }
 
-// Wavefront-cooperative argmax over one hull.  All 64 lanes call this with
-// the same (wave-uniform) arguments.  Each lane scans vertices lane, lane+64,
-// ... of the FP32 SoA pool (coalesced 256-byte rows), tracking the largest
-// score m1 (first index i1) and the largest score m2 among all other
-// vertices; a butterfly reduction merges the lanes.  The FP32 score of vertex
-// v differs from the exact v . l by at most
+__device__ __forceinline__ HullRef readlane_hull(const HullRef &H, int lane) {
+    return HullRef{readlane_i32(H.voff, lane), readlane_i32(H.voff64, lane),
+                   readlane_i32(H.nv, lane),   readlane_i32(H.coff, lane),
+                   readlane_i32(H.nc, lane),   readlane_f64(H.L, lane)};
+}
+
+// Bits of a wavefront ballot that belong to the G-lane group of `lane`.
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long m, int lane) {
+    if (G == kWave) return m;
+    return (m >> (lane & ~(G - 1))) & ((1ull << G) - 1ull);
+}
+
+template <int G>
+__device__ __forceinline__ float group_maxf(float x) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off));
+    return x;
+}
+
+// One FP32 score into the running (largest m1 at i1, largest other m2).
+__device__ __forceinline__ void top2(float sc, int j, float &m1, float &m2, int &i1) {
+    m2 = fmaxf(m2, fminf(sc, m1));
+    if (sc > m1) {
+        m1 = sc;
+        i1 = j;
+    }
+}
+
+// Argmax over one hull, vertices[argmax(vertices . l)] with the first
+// (original) index on ties (colliders.py:131-132, np.argmax), evaluated by
+// the G lanes of a group (G a power of two <= 64; all lanes of the group
+// call it with the same arguments, `sub` = lane index in the group).
+//
+// FP32 sweep with an exactness certificate.  The FP32 score of vertex v
+// differs from the exact v . l by at most
 //     E = 8 * 2^-24 * L * |l|_inf,   L = max_i (|x_i| + |y_i| + |z_i|)
 // (rounding of l to FP32, of the vertex when an FP64 pool backs it, and of
-// the three FMA steps, with a factor-of-two margin).  If m1 - m2 > 2E, vertex
-// i1 is the unique exact argmax, hence also the FP64 argmax.  Otherwise the
-// hull is rescanned in FP64 with the serial rule (first index on ties).
-__device__ __forceinline__ int coop_hull_argmax(const d3d_scene &s, int voff, int voff64, int nv,
-                                                double L, v3 l) {
-    const int lane = lane_id();
+// the three FMA steps, with a factor-of-two margin).  The lanes track the
+// largest score m1 (vertex i1) and the largest score m2 among all other
+// scanned vertices; if m1 - m2 > 2E after the group merge, i1 is the unique
+// exact argmax.  Otherwise the hull is rescanned in FP64 and ties go to the
+// smallest original index (vorig).
+//
+// Clustered hulls (nc > 0) are stored as consecutive clusters of
+// D3D_CLUSTER_SIZE vertices, each with a sphere (c, r) such that
+// fp32(c . d) + r * |l| (one FMA chain, |l| rounded up) is at least the
+// exact score of every vertex of the cluster.  The sweep scans the cluster
+// with the largest bound first, then only the clusters whose bound is not
+// below m1 - 2.5E: a skipped vertex has exact score below m1 - 2.5E, hence
+// below the exact score of the winner (>= m1 - E), so the certificate keeps
+// its meaning.  The result equals the full sweep's bit for bit.
+template <int G>
+__device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H, v3 l, int sub) {
+    constexpr int K = D3D_CLUSTER_SIZE;
     const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
-    const float *__restrict__ px = s.vx + voff;
-    const float *__restrict__ py = s.vy + voff;
-    const float *__restrict__ pz = s.vz + voff;
+    const float *__restrict__ px = s.vx + H.voff;
+    const float *__restrict__ py = s.vy + H.voff;
+    const float *__restrict__ pz = s.vz + H.voff;
+    const int nv = H.nv;
+    const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
+    const double E = 8.0 * 5.9604644775390625e-08 * H.L * linf;
     float m1 = -INFINITY, m2 = -INFINITY;
     int i1 = 0x7fffffff;
-    int k = lane;
-    for (; k + kWave < nv; k += 2 * kWave) {
-        const float x0 = px[k], y0 = py[k], z0 = pz[k];
-        const float x1 = px[k + kWave], y1 = py[k + kWave], z1 = pz[k + kWave];
-        const float s0 = __builtin_fmaf(z0, dz, __builtin_fmaf(y0, dy, x0 * dx));
-        const float s1 = __builtin_fmaf(z1, dz, __builtin_fmaf(y1, dy, x1 * dx));
-        m2 = fmaxf(m2, fminf(s0, m1));
-        if (s0 > m1) { m1 = s0; i1 = k; }
-        m2 = fmaxf(m2, fminf(s1, m1));
-        if (s1 > m1) { m1 = s1; i1 = k + kWave; }
-    }
-    if (k < nv) {
-        const float s0 = __builtin_fmaf(pz[k], dz, __builtin_fmaf(py[k], dy, px[k] * dx));
-        m2 = fmaxf(m2, fminf(s0, m1));
-        if (s0 > m1) { m1 = s0; i1 = k; }
+    auto score = [&](int j) {
+        return __builtin_fmaf(pz[j], dz, __builtin_fmaf(py[j], dy, px[j] * dx));
+    };
+    if (H.nc == 0) {
+        // strided sweep, four vertices per lane per step (loads issued together)
+        int k = sub;
+        for (; k + 3 * G < nv; k += 4 * G) {
+            const float s0 = score(k), s1 = score(k + G), s2 = score(k + 2 * G), s3 = score(k + 3 * G);
+            top2(s0, k, m1, m2, i1);
+            top2(s1, k + G, m1, m2, i1);
+            top2(s2, k + 2 * G, m1, m2, i1);
+            top2(s3, k + 3 * G, m1, m2, i1);
+        }
+        for (; k < nv; k += G) top2(score(k), k, m1, m2, i1);
+    } else {
+        const float4 *__restrict__ cl = reinterpret_cast<const float4 *>(s.clusters) + H.coff;
+        const int nc = H.nc;
+        const float nl = (float)(sqrt(l.x * l.x + l.y * l.y + l.z * l.z) * (1.0 + 0x1p-19));
+        auto bound = [&](int k) {
+            const float4 c = cl[k];
+            return __builtin_fmaf(c.w, nl, __builtin_fmaf(c.z, dz, __builtin_fmaf(c.y, dy, c.x * dx)));
+        };
+        // the cluster with the largest bound (first on ties)
+        float bu = -INFINITY;
+        int bk = 0x7fffffff;
+#pragma unroll 4
+        for (int k = sub; k < nc; k += G) {
+            const float ub = bound(k);
+            if (ub > bu) {
+                bu = ub;
+                bk = k;
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            const float ou = __shfl_xor(bu, off);
+            const int ok = __shfl_xor(bk, off);
+            const bool take = (ou > bu) || (ou == bu && ok < bk);
+            bu = take ? ou : bu;
+            bk = take ? ok : bk;
+        }
+        const int ks = bk < nc ? bk : 0;
+        // scan of one cluster by the group: lane sub takes slots sub, sub+G, ...
+        auto scan1 = [&](int j0) {
+            const int cnt = min(K, nv - j0);
+#pragma unroll
+            for (int o = sub % K; o < K; o += (G < K ? G : K))
+                if (o < cnt) top2(score(j0 + o), j0 + o, m1, m2, i1);
+        };
+        if (G <= K || sub < K) scan1(ks * K);
+        float thr = __double2float_rd((double)group_maxf<G>(m1) - 2.5 * E);
+        const int lane = lane_id();
+        for (int base = 0; base < nc; base += G) {
+            const int k = base + sub;
+            bool surv = false;
+            if (k < nc && k != ks) surv = !(bound(k) < thr);
+            unsigned long long m = (G == 1) ? (unsigned long long)surv : group_bits<G>(__ballot(surv), lane);
+            if (G <= K) {
+                // two clusters per step when a lane holds few slots of each
+                constexpr bool kTwo = (K / (G < K ? G : K)) <= 4;
+                while (m) {
+                    const int j0 = (base + __ffsll((long long)m) - 1) * K;
+                    m &= m - 1;
+                    if (kTwo && m) {
+                        const int j1 = (base + __ffsll((long long)m) - 1) * K;
+                        m &= m - 1;
+                        const int c0 = min(K, nv - j0), c1 = min(K, nv - j1);
+#pragma unroll
+                        for (int o = sub; o < K; o += (G < K ? G : K)) {
+                            const float s0 = o < c0 ? score(j0 + o) : -INFINITY;
+                            const float s1 = o < c1 ? score(j1 + o) : -INFINITY;
+                            if (o < c0) top2(s0, j0 + o, m1, m2, i1);
+                            if (o < c1) top2(s1, j1 + o, m1, m2, i1);
+                        }
+                    } else {
+                        scan1(j0);
+                    }
+                }
+            } else {
+                // G / K clusters per step, one per K-lane subgroup
+                constexpr int P = G > K ? G / K : 1;
+                const int q = sub / K, r = sub % K;
+                while (m) {
+                    unsigned long long mm = m;
+                    for (int t = 0; t < q; ++t) mm &= mm - 1;
+                    if (mm) {
+                        const int j = (base + __ffsll((long long)mm) - 1) * K + r;
+                        if (j < nv) top2(score(j), j, m1, m2, i1);
+                    }
+#pragma unroll
+                    for (int t = 0; t < P; ++t) m &= m - 1;
+                }
+            }
+            if (base + G < nc) thr = __double2float_rd((double)group_maxf<G>(m1) - 2.5 * E);
+        }
     }
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
+    for (int off = G / 2; off > 0; off >>= 1) {
         const float om1 = __shfl_xor(m1, off);
         const float om2 = __shfl_xor(m2, off);
         const int oi = __shfl_xor(i1, off);
@@ -237,27 +362,29 @@ __device__ __forceinline__ int coop_hull_argmax(const d3d_scene &s, int voff, in
         m1 = take ? om1 : m1;
         i1 = take ? oi : i1;
     }
-    const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
-    const double E = 8.0 * 5.9604644775390625e-08 * L * linf;
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
 
     double best = -INFINITY;
-    int bi = 0x7fffffff;
-    for (int j = lane; j < nv; j += kWave) {
-        v3 v = hull_vertex(s, voff, voff64, j);
-        double val = v.x * l.x + v.y * l.y + v.z * l.z;
-        if (val > best) {
+    int bi = 0x7fffffff, bo = 0x7fffffff;
+    for (int j = sub; j < nv; j += G) {
+        const v3 v = hull_vertex(s, H.voff, H.voff64, j);
+        const double val = v.x * l.x + v.y * l.y + v.z * l.z;
+        const int oj = s.vorig ? s.vorig[H.voff + j] : j;
+        if (val > best || (val == best && oj < bo)) {
             best = val;
             bi = j;
+            bo = oj;
         }
     }
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
+    for (int off = G / 2; off > 0; off >>= 1) {
         const double ob = __shfl_xor(best, off);
         const int oi = __shfl_xor(bi, off);
-        const bool take = (ob > best) || (ob == best && oi < bi);
+        const int oo = __shfl_xor(bo, off);
+        const bool take = (ob > best) || (ob == best && oo < bo);
         best = take ? ob : best;
         bi = take ? oi : bi;
+        bo = take ? oo : bo;
     }
     return bi == 0x7fffffff ? 0 : bi;
 }
@@ -357,6 +484,38 @@ __device__ __forceinline__ v3 apply_margin(const Shape &h, v3 p, v3 d) {
     return p;
 }
 
+// Supports of record ia along d and of record ib along -d (the two halves of
+// a Minkowski-difference support), computed one collider after the other so
+// that a single copy of the sweep is compiled.  The G lanes of a group call it
+// together with the same arguments; records are reloaded by index and must
+// have been validated (load_checked) by the caller.
+struct SupportPair {
+    v3 a, b;
+};
+
+template <int G, bool kHull>
+__device__ __forceinline__ SupportPair support_pair(const d3d_scene &s, int ia, int ib, v3 d,
+                                                    int sub) {
+    v3 a = mk(0.0, 0.0, 0.0), b = a;
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const Shape h = load_shape(s, opaque(k == 0 ? ia : ib));
+        const v3 dir = vsel(k == 0, d, -d);
+        v3 out;
+        if (kHull && h.type == D3D_HULL) {
+            const v3 l = to_local(h, dir);
+            const int idx = hull_argmax<G>(s, hull_ref(h), l, sub);
+            out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));
+        } else {
+            out = support_analytic(h, dir);
+        }
+        out = apply_margin(h, out, dir);
+        a = vsel(k == 0, out, a);
+        b = vsel(k == 0, b, out);
+    }
+    return SupportPair{a, b};
+}
+
 // Support of one collider per lane along d (per lane).  Hulls with more than
 // serial_max vertices are swept by the whole wavefront, one lane's hull at a
 // time.  Must be called by all 64 lanes (wave-uniform control flow); lanes
@@ -370,21 +529,18 @@ __device__ __forceinline__ v3 support_wave(const d3d_scene &s, const Shape &h, v
     if (want && h.type == D3D_HULL) l = to_local(h, d);
     if (want && !coop) {
         if (h.type == D3D_HULL) {
-            idx = serial_hull_argmax(s, h.voff, h.voff64, h.nv, l);
+            idx = hull_argmax<1>(s, hull_ref(h), l, 0);
         } else {
             out = support_analytic(h, d);
         }
     }
     unsigned long long m = __ballot(coop);
+    const HullRef H = hull_ref(h);
     while (m) {
         const int j = __ffsll((long long)m) - 1;
         m &= m - 1;
-        const int voff = readlane_i32(h.voff, j);
-        const int voff64 = readlane_i32(h.voff64, j);
-        const int nv = readlane_i32(h.nv, j);
-        const double L = readlane_f64(h.p[0], j);
         const v3 lj = mk(readlane_f64(l.x, j), readlane_f64(l.y, j), readlane_f64(l.z, j));
-        const int best = coop_hull_argmax(s, voff, voff64, nv, L, lj);
+        const int best = hull_argmax<kWave>(s, readlane_hull(H, j), lj, lane_id());
         if (lane_id() == j) idx = best;
     }
     if (want && h.type == D3D_HULL) out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));

@@ -6,7 +6,7 @@
 // and l + 64 in every face-parallel step.  Each iteration:
 //   closest face    wavefront argmin over face distances (first index on ties)
 //   support         both colliders along the face normal (hull sweeps use all
-//                   64 lanes, see coop_hull_argmax)
+//                   64 lanes, see hull_argmax)
 //   visible faces   per-face test n . (w - v0) > eps
 //   horizon         edges of removed faces whose reverse edge is not on
 //                   another removed face (vertex-index matching)
@@ -33,19 +33,6 @@ struct EpaLds {
     int fv0[kEpaMaxFaces], fv1[kEpaMaxFaces], fv2[kEpaMaxFaces];
     int rv0[kEpaMaxFaces], rv1[kEpaMaxFaces], rv2[kEpaMaxFaces];
 };
-
-// Support of a wave-uniform collider along a wave-uniform direction.
-__device__ __forceinline__ v3 support_uniform(const d3d_scene &s, const Shape &h, v3 d) {
-    v3 out;
-    if (h.type == D3D_HULL) {
-        const v3 l = to_local(h, d);
-        const int idx = coop_hull_argmax(s, h.voff, h.voff64, h.nv, h.p[0], l);
-        out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));
-    } else {
-        out = support_analytic(h, d);
-    }
-    return apply_margin(h, out, d);
-}
 
 __device__ __forceinline__ v3 lds_w(const EpaLds &L, int v) { return mk(L.wx[v], L.wy[v], L.wz[v]); }
 
@@ -77,7 +64,7 @@ struct SeedVerts {
 
 // Seed tetrahedron containing the origin from a GJK simplex of n points
 // (uniform across the wavefront).  Returns false for a degenerate seed.
-__device__ bool epa_seed(const d3d_scene &s, const Shape &SA, const Shape &SB, SeedVerts &V, int n) {
+__device__ bool epa_seed(const d3d_scene &s, int ia, int ib, SeedVerts &V, int n) {
     if (n < 1 || n > 4) return false;
     if (n == 1) {
         int best = -1;
@@ -86,7 +73,8 @@ __device__ bool epa_seed(const d3d_scene &s, const Shape &SA, const Shape &SB, S
         for (int k = 0; k < 6; ++k) {
             const double sg = (k & 1) ? -1.0 : 1.0;
             const v3 d = mk(k < 2 ? sg : 0.0, (k >= 2 && k < 4) ? sg : 0.0, k >= 4 ? sg : 0.0);
-            const v3 pa = support_uniform(s, SA, d), pb = support_uniform(s, SB, -d);
+            const SupportPair sp = support_pair<kWave, true>(s, ia, ib, d, lane_id());
+            const v3 pa = sp.a, pb = sp.b;
             const v3 w = pa - pb;
             const v3 e = w - V.W[0];
             const double l = dot(e, e);
@@ -116,7 +104,8 @@ __device__ bool epa_seed(const d3d_scene &s, const Shape &SA, const Shape &SB, S
         v3 bw = V.W[0], bpa = V.A[0], bpb = V.B[0];
         for (int k = 0; k < 4; ++k) {
             const v3 c = sel4(k, p1, -p1, p2, -p2);
-            const v3 pa = support_uniform(s, SA, c), pb = support_uniform(s, SB, -c);
+            const SupportPair sp = support_pair<kWave, true>(s, ia, ib, c, lane_id());
+            const v3 pa = sp.a, pb = sp.b;
             const v3 w = pa - pb;
             const v3 cr = cross(ab, w - V.W[0]);
             const double l = dot(cr, cr);
@@ -137,9 +126,11 @@ __device__ bool epa_seed(const d3d_scene &s, const Shape &SA, const Shape &SB, S
     if (n == 3) {
         const v3 nrm = cross(V.W[1] - V.W[0], V.W[2] - V.W[0]);
         if (!(dot(nrm, nrm) > 0.0)) return false;
-        const v3 wp_pa = support_uniform(s, SA, nrm), wp_pb = support_uniform(s, SB, -nrm);
+        const SupportPair sp = support_pair<kWave, true>(s, ia, ib, nrm, lane_id());
+        const v3 wp_pa = sp.a, wp_pb = sp.b;
         const v3 wp = wp_pa - wp_pb;
-        const v3 wm_pa = support_uniform(s, SA, -nrm), wm_pb = support_uniform(s, SB, nrm);
+        const SupportPair sm = support_pair<kWave, true>(s, ia, ib, -nrm, lane_id());
+        const v3 wm_pa = sm.a, wm_pb = sm.b;
         const v3 wm = wm_pa - wm_pb;
         const double hp = dot(wp - V.W[0], nrm), hm = -dot(wm - V.W[0], nrm);
         if (hp >= hm) {
@@ -167,7 +158,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
     const int64_t count = n_dev ? *n_dev : n;
     for (int64_t item = blockIdx.x; item < count; item += gridDim.x) {
         const int64_t pid = work ? (int64_t)work[item] : item;
-        Shape SA, SB;
+        Shape SA, SB;  // validation only; supports reload the records by index
         const bool okA = load_checked(s, pa[pid], SA);
         const bool okB = load_checked(s, pb[pid], SB);
         double *o = out + 13 * pid;
@@ -190,7 +181,8 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             V.B[r] = mk(S[24 + 3 * r], S[25 + 3 * r], S[26 + 3 * r]);
         }
         int status = 0;
-        bool seed_ok = epa_seed(s, SA, SB, V, npts[pid]);
+        const int ia = pa[pid], ib = pb[pid];
+        bool seed_ok = epa_seed(s, ia, ib, V, npts[pid]);
         int nf = 0;
         if (seed_ok) {
             const v3 nrm = cross(V.W[1] - V.W[0], V.W[2] - V.W[0]);
@@ -266,7 +258,8 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             kb = bk;
             const v3 cn = mk(L.fnx[kb], L.fny[kb], L.fnz[kb]);
             const double cdist = L.fd[kb];
-            const v3 spa = support_uniform(s, SA, cn), spb = support_uniform(s, SB, -cn);
+            const SupportPair sp = support_pair<kWave, true>(s, ia, ib, cn, lane);
+            const v3 spa = sp.a, spb = sp.b;
             const v3 w = spa - spb;
             const double wn = dot(w, cn);
             if (wn - cdist < eps) {

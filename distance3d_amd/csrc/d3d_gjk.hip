@@ -7,7 +7,7 @@
 // (Y, P, Q of up to four points) lives in VGPRs of its lane; support points of
 // analytic colliders and of small hulls are computed by the lane itself, while
 // large hulls are swept by the whole wavefront (coalesced FP32 SoA rows, see
-// coop_hull_argmax), one requesting lane at a time.  The loop runs until every
+// hull_argmax), one requesting lane at a time.  The loop runs until every
 // lane of the wavefront has terminated; finished lanes keep helping with the
 // sweeps of the others.
 #include "d3d_device.hpp"

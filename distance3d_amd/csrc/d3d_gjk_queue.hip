@@ -361,83 +361,6 @@ __device__ __forceinline__ void gjk_write_bad(int64_t pid, const GjkOut &o) {
 
 // ------------------------------------------------------- group supports
 
-// Argmax over one hull by the G lanes of a group (G a power of two <= 64):
-// lane `sub` scans vertices sub, sub + G, ...; the FP32 scores carry the
-// exactness certificate of coop_hull_argmax, with an FP64 rescan by the
-// group when it fails.
-template <int G>
-__device__ __forceinline__ int group_hull_argmax(const d3d_scene &s, int voff, int voff64, int nv,
-                                                 double L, v3 l, int sub) {
-    const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
-    const float *__restrict__ px = s.vx + voff;
-    const float *__restrict__ py = s.vy + voff;
-    const float *__restrict__ pz = s.vz + voff;
-    float m1 = -INFINITY, m2 = -INFINITY;
-    int i1 = 0x7fffffff;
-    int k = sub;
-    for (; k + G < nv; k += 2 * G) {
-        const float x0 = px[k], y0 = py[k], z0 = pz[k];
-        const float x1 = px[k + G], y1 = py[k + G], z1 = pz[k + G];
-        const float s0 = __builtin_fmaf(z0, dz, __builtin_fmaf(y0, dy, x0 * dx));
-        const float s1 = __builtin_fmaf(z1, dz, __builtin_fmaf(y1, dy, x1 * dx));
-        m2 = fmaxf(m2, fminf(s0, m1));
-        if (s0 > m1) { m1 = s0; i1 = k; }
-        m2 = fmaxf(m2, fminf(s1, m1));
-        if (s1 > m1) { m1 = s1; i1 = k + G; }
-    }
-    if (k < nv) {
-        const float s0 = __builtin_fmaf(pz[k], dz, __builtin_fmaf(py[k], dy, px[k] * dx));
-        m2 = fmaxf(m2, fminf(s0, m1));
-        if (s0 > m1) { m1 = s0; i1 = k; }
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        const float om1 = __shfl_xor(m1, off);
-        const float om2 = __shfl_xor(m2, off);
-        const int oi = __shfl_xor(i1, off);
-        m2 = fmaxf(fmaxf(m2, om2), fminf(m1, om1));
-        const bool take = (om1 > m1) || (om1 == m1 && oi < i1);
-        m1 = take ? om1 : m1;
-        i1 = take ? oi : i1;
-    }
-    const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
-    const double E = 8.0 * 5.9604644775390625e-08 * L * linf;
-    if ((double)m1 - (double)m2 > 2.0 * E) return i1;
-
-    double best = -INFINITY;
-    int bi = 0x7fffffff;
-    for (int j = sub; j < nv; j += G) {
-        const v3 v = hull_vertex(s, voff, voff64, j);
-        const double val = v.x * l.x + v.y * l.y + v.z * l.z;
-        if (val > best) {
-            best = val;
-            bi = j;
-        }
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        const bool take = (ob > best) || (ob == best && oi < bi);
-        best = take ? ob : best;
-        bi = take ? oi : bi;
-    }
-    return bi == 0x7fffffff ? 0 : bi;
-}
-
-template <int G, bool kHull>
-__device__ __forceinline__ v3 support_group(const d3d_scene &s, const Shape &h, v3 d, int sub) {
-    v3 out;
-    if (kHull && h.type == D3D_HULL) {
-        const v3 l = to_local(h, d);
-        const int idx = group_hull_argmax<G>(s, h.voff, h.voff64, h.nv, h.p[0], l, sub);
-        out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));
-    } else {
-        out = support_analytic(h, d);
-    }
-    return apply_margin(h, out, d);
-}
-
 __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 }
@@ -503,11 +426,8 @@ __global__ __launch_bounds__(kQBlock) void gjk_queue_kernel(
                 running = false;
             } else {
                 // the records were validated when the pair was taken
-                const Shape A = load_shape(s, opaque(ia));
-                const v3 p = support_group<G, kHull>(s, A, g.d, sub);
-                const Shape B = load_shape(s, opaque(ib));
-                const v3 q = support_group<G, kHull>(s, B, -g.d, sub);
-                running = gjk_distance_step(g, pq, p, q, prm.max_distance_squared, tol2);
+                const SupportPair w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
+                running = gjk_distance_step(g, pq, w.a, w.b, prm.max_distance_squared, tol2);
             }
             if (!running) {
                 if (sub == 0) gjk_write(g, pq, pid, prm.sanity_check, o);

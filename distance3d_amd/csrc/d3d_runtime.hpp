@@ -57,6 +57,10 @@ inline int check_scene(const d3d_scene *s) {
         set_error("scene has FP64 vertices but a NULL v64 pointer");
         return D3D_E_INVALID;
     }
+    if (s->n_clusters < 0 || (s->n_clusters > 0 && !s->clusters)) {
+        set_error("scene has clusters but a NULL cluster pointer");
+        return D3D_E_INVALID;
+    }
     return D3D_OK;
 }
 

@@ -5,6 +5,7 @@ import numpy as np
 
 from . import _native
 from ._native import check, lib, ptr, stream_ptr
+from .scene import scene_descriptor
 
 
 def _dev_i32(x, device):
@@ -235,9 +236,7 @@ class SceneView:
         self.shape_f64 = shape_f64
         self.shape_i32 = shape_i32
         self.n_shapes = n_shapes
-        self._c = _native.d3d_scene(
-            shape_f64.data_ptr(), shape_i32.data_ptr(), base.vx.data_ptr(), base.vy.data_ptr(),
-            base.vz.data_ptr(), base.v64.data_ptr(), n_shapes, base.n_vertices, base.n_vertices64)
+        self._c = scene_descriptor(base, shape_f64, shape_i32, n_shapes)
 
     @property
     def c(self):

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define D3D_ABI_VERSION 1
+#define D3D_ABI_VERSION 2
 
 /* error codes */
 #define D3D_OK 0
@@ -57,7 +57,9 @@ extern "C" {
  *   shape_f64[k*16 + 0..8]  R = collider2origin[:3,:3], row major
  *   shape_f64[k*16 + 9..11] t = collider2origin[:3,3]
  *   shape_f64[k*16 + 12..14] type parameters:
- *       HULL: p0 = max_i(|x_i|+|y_i|+|z_i|) over the local vertices
+ *       HULL: p0 = max_i(|x_i|+|y_i|+|z_i|) over the local vertices,
+ *             p1 = index of the hull's first cluster record, p2 = number of
+ *             clusters (0: unclustered; otherwise ceil(n/D3D_CLUSTER_SIZE))
  *       SPHERE: radius; CAPSULE: radius, height; CYLINDER: radius, length;
  *       CONE: radius, height; DISK: radius (R columns = plane basis, normal);
  *       ELLIPSE: radius0, radius1 (R columns 0,1 = axes); BOX: size[3];
@@ -69,7 +71,20 @@ extern "C" {
  *   v64                     FP64 vertex pool, (n, 3) row major, for hulls
  *                           whose vertices are not exactly representable in
  *                           FP32 (may be NULL when no shape uses it)
+ *   vorig                   original index of each FP32-pool vertex within
+ *                           its hull (NULL: pool order is the original order);
+ *                           ties of the support argmax go to the smallest
+ *   clusters                (n_clusters, 4) FP32 records {cx, cy, cz, r} of a
+ *                           clustered hull: its vertices j*K .. j*K+K-1
+ *                           (K = D3D_CLUSTER_SIZE, in pool order) lie within
+ *                           the sphere, inflated so that the FP32 bound
+ *                           c.d + r|d| of the sweep is never below an exact
+ *                           vertex score (may be NULL when n_clusters == 0)
+ *
+ * d3d_cluster_scene builds a clustered copy of a scene on the host.
  */
+#define D3D_CLUSTER_SIZE 16
+
 typedef struct d3d_scene {
     const double *shape_f64;
     const int32_t *shape_i32;
@@ -80,6 +95,9 @@ typedef struct d3d_scene {
     int64_t n_shapes;
     int64_t n_vertices;
     int64_t n_vertices64;
+    const int32_t *vorig;
+    const float *clusters;
+    int64_t n_clusters;
 } d3d_scene;
 
 /* gjk_distance_jolt keyword arguments (gjk/_gjk_jolt.py:138-139) plus the
@@ -107,6 +125,29 @@ int d3d_abi_version(void);
 const char *d3d_last_error(void);
 /* Selects `device` for the calling thread and checks it is a gfx950 GPU. */
 int d3d_init(int device);
+
+/* ---- scene layout for pruned support sweeps (HOST memory, no GPU needed) --
+ * Replaces nothing in the reference: it reorders each ConvexHullVertices
+ * vertex set (colliders.py:109-144) into spatial clusters so the argmax of
+ * support_function (colliders.py:131-132) can skip clusters whose bound lies
+ * below the running maximum; results are unchanged bit for bit.
+ *
+ * d3d_cluster_plan: output sizes for d3d_cluster_scene.  Hulls with at least
+ * min_vertices vertices (min_vertices > 0) are clustered.
+ * d3d_cluster_scene: writes the clustered scene.  Inputs are a host scene in
+ * the layout above (unclustered: p1 = p2 = 0, vorig NULL); outputs are host
+ * arrays sized by the plan: out_f64 (n_shapes,16), out_i32 (n_shapes,4),
+ * ovx/ovy/ovz (n_vertices), ov64 (n_vertices64,3), vorig (n_vertices),
+ * clusters (n_clusters,4).  Hull vertex ranges are re-laid out (a clustered
+ * hull starts at a multiple of D3D_CLUSTER_SIZE); records sharing one vertex
+ * range keep sharing it.  n_threads <= 0 uses all hardware threads. */
+int d3d_cluster_plan(const int32_t *shape_i32, int64_t n_shapes, int32_t min_vertices,
+                     int64_t *n_vertices, int64_t *n_vertices64, int64_t *n_clusters);
+int d3d_cluster_scene(const double *shape_f64, const int32_t *shape_i32, int64_t n_shapes,
+                      const float *vx, const float *vy, const float *vz, const double *v64,
+                      int64_t n_vertices, int64_t n_vertices64, int32_t min_vertices,
+                      int32_t n_threads, double *out_f64, int32_t *out_i32, float *ovx,
+                      float *ovy, float *ovz, double *ov64, int32_t *vorig, float *clusters);
 
 /*
  * Support points, collider.support_function(d) (colliders.py:131-132,

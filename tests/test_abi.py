@@ -31,7 +31,7 @@ def test_library_exports_declared_symbols():
     lib = _native.load_library()
     for name in sorted(_declared()):
         assert hasattr(lib, name), name
-    assert lib.d3d_abi_version() == 1
+    assert lib.d3d_abi_version() == _native.ABI_VERSION
 
 
 def test_binding_covers_header():
