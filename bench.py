@@ -44,12 +44,32 @@ def parse():
     p.add_argument("--no-extra", action="store_true", help="only the headline workload")
     p.add_argument("--only", type=str, default=None,
                    help="comma list of extra configs: hull,epa,broad,serving")
-    p.add_argument("--cpu-runs", type=int, default=3)
+    p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--serving-requests", type=int, default=50000)
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="before the workload, run one device-to-device copy of this many MiB "
                         "(a kernel of known traffic for scaling profiler byte counters)")
     return p.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "traffic_mixed_pairs.json")
+
+
+def measured_traffic():
+    """HBM bytes per step of the headline workload as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in profiles/ (scaled by
+    the copy-kernel calibration stated there), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    return t
+
+
+def traffic_bytes():
+    t = measured_traffic()
+    return None if t is None else t["bytes_per_step"]
 
 
 def log(msg):
@@ -147,28 +167,78 @@ def run_pipeline_config(w, args, lanes, steps, warmup, label, parity=True, cpu=F
                                               out.epa_status.cpu().numpy()[idx], e)
         log("%s parity in %.1fs" % (label, time.time() - t0))
     if cpu:
-        from oracle import oracle
-        cores = oracle.num_threads()
+        res["cpu_baseline"], res["cpu_single_core"] = cpu_baseline(w, args)
+    return res
 
-        def cpu_step():
-            o = oracle.gjk_distance(w.scene, w.pair_a, w.pair_b, want_simplex=True)
-            idx = np.nonzero(o["dist"] == 0.0)[0]
-            if len(idx):
-                oracle.epa(w.scene, w.pair_a[idx], w.pair_b[idx], o["simplex"][idx],
-                           o["n_points"][idx])
-        sub = min(n, 65536)
-        oracle.gjk_distance(w.scene, w.pair_a[:sub], w.pair_b[:sub])
+
+def cpu_baseline(w, args):
+    """The oracle (C FP64 restatement, OpenMP over pairs) on this host: all
+    threads over the whole workload, 1 warm-up + median of ``--cpu-runs``; and
+    one thread over the first 131,072 pairs, median of 3."""
+    from oracle import oracle
+
+    def cpu_step(pa, pb):
+        o = oracle.gjk_distance(w.scene, pa, pb, want_simplex=True)
+        idx = np.nonzero(o["dist"] == 0.0)[0]
+        if len(idx):
+            oracle.epa(w.scene, pa[idx], pb[idx], o["simplex"][idx], o["n_points"][idx])
+
+    def timed(pa, pb, runs):
+        cpu_step(pa, pb)
         times = []
-        for _ in range(args.cpu_runs):
+        for _ in range(runs):
             t0 = time.perf_counter()
-            cpu_step()
+            cpu_step(pa, pb)
             times.append(time.perf_counter() - t0)
-        rate = n / float(np.median(times))
-        res["cpu_baseline"] = {
-            "value": round(rate, 1), "unit": "pair-queries/s", "cores": cores, "kind": "port",
+        return len(pa) / float(np.median(times)), sum(times)
+
+    cores = oracle.num_threads()
+    rate, spent = timed(w.pair_a, w.pair_b, args.cpu_runs)
+    full = {"value": round(rate, 1), "unit": "pair-queries/s", "cores": cores, "kind": "port",
             "sample": "all %d pairs of the same workload (GJK + EPA on intersecting pairs), "
-                      "median of %d runs, %.1fs CPU wall in total; C FP64 restatement, OpenMP"
-                      % (n, args.cpu_runs, sum(times))}
+                      "1 warm-up + median of %d runs (%.1f s); C FP64 restatement, OpenMP "
+                      "dynamic schedule over pairs" % (w.n_pairs, args.cpu_runs, spent)}
+    sub = min(w.n_pairs, 131072)
+    oracle.set_threads(1)
+    try:
+        rate1, spent1 = timed(w.pair_a[:sub], w.pair_b[:sub], 3)
+    finally:
+        oracle.set_threads(cores)
+    single = {"value": round(rate1, 1), "unit": "pair-queries/s", "cores": 1, "kind": "port",
+              "sample": "first %d pairs, 1 warm-up + median of 3 runs (%.1f s)" % (sub, spent1)}
+    log("cpu baseline: %.3g pairs/s on %d threads, %.3g pairs/s on 1" % (rate, cores, rate1))
+    return full, single
+
+
+def one_pair_per_launch(args, n_calls=2000):
+    """The same library entry point called with one pair per launch and a
+    synchronisation after each call, over the 4096 hull pairs of the 32-vertex
+    workload (the plain per-pair GPU path)."""
+    import torch
+    from distance3d_amd import kernels, workloads
+    w = workloads.hull_pairs()
+    dev = w.scene.to_device("cuda:0")
+    pa = torch.from_numpy(w.pair_a).cuda()
+    pb = torch.from_numpy(w.pair_b).cuda()
+    out = kernels.GjkResult(torch.empty(1, dtype=torch.float64, device="cuda"),
+                            torch.empty((1, 3), dtype=torch.float64, device="cuda"),
+                            torch.empty((1, 3), dtype=torch.float64, device="cuda"),
+                            torch.empty(1, dtype=torch.int32, device="cuda"), None, None, None)
+    ws = torch.empty(kernels.gjk_workspace_bytes(1), dtype=torch.uint8, device="cuda")
+    n = w.n_pairs
+    for i in range(20):
+        kernels.gjk_distance_batch(dev, pa[i:i + 1], pb[i:i + 1], out=out, workspace=ws)
+        torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_calls):
+        k = i % n
+        kernels.gjk_distance_batch(dev, pa[k:k + 1], pb[k:k + 1], out=out, workspace=ws)
+        torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"value": round(n_calls / dt, 1), "unit": "pair-queries/s",
+            "sample": "%d calls of d3d_gjk_distance_batch_ws with one pair each, synchronised "
+                      "after every call, cycling over the 4096 32-vertex hull pairs" % n_calls}
+
     return res
 
 
@@ -312,7 +382,7 @@ def main():
                                "intersecting pairs",
                    "pairs": w.n_pairs, "seed": args.seed, "hull_vertices": "log-uniform 8..4096",
                    "lanes_per_pair": lanes or list(_native.DEFAULT_LANES)},
-        "roofline": dict(head["roofline"], traffic=None,
+        "roofline": dict(head["roofline"], traffic=traffic_bytes(),
                          kernel="gjk_epa pipeline step (sort + 8 work-class GJK kernels + "
                                 "compaction + EPA)", kernel_ms=head["event_ms_per_step"]),
     }
@@ -320,7 +390,15 @@ def main():
         result["parity"] = head["parity"]
     if "cpu_baseline" in head:
         result["cpu_baseline"] = head["cpu_baseline"]
+        result["cpu_single_core"] = head["cpu_single_core"]
         result["speedup_vs_cpu"] = round(head["value"] / head["cpu_baseline"]["value"], 2)
+        result["speedup_vs_cpu_single_core"] = round(
+            head["value"] / head["cpu_single_core"]["value"], 2)
+    if not args.no_extra:
+        opl = one_pair_per_launch(args)
+        result["one_pair_per_launch"] = opl
+        result["speedup_vs_one_pair_per_launch"] = round(head["value"] / opl["value"], 1)
+        log("one pair per launch: %.3g pairs/s" % opl["value"])
     if not args.no_extra:
         only = set(args.only.split(",")) if args.only else {"hull", "epa", "broad", "serving"}
         cfgs = {}

@@ -20,8 +20,8 @@ from . import _native
 
 TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 CLASS_EDGES = (0, 32, 128, 512, 1024, 2048, 4096)
-CANDIDATES = {1: (1, 2), 2: (1, 2, 4), 3: (1, 2, 4, 8), 4: (1, 2, 4, 8), 5: (2, 4, 8, 16),
-              6: (4, 8, 16, 32), 7: (8, 16, 32, 64)}
+CANDIDATES = {0: (1, 2, 4), 1: (1, 2, 4), 2: (1, 2, 4, 8), 3: (2, 4, 8, 16), 4: (2, 4, 8, 16),
+              5: (4, 8, 16, 32), 6: (4, 8, 16, 32, 64), 7: (8, 16, 32, 64)}
 
 
 def work_class(nverts):
@@ -75,7 +75,7 @@ def tune(workload, reps=3, log=print):
     cls = pair_classes(workload.scene, workload.pair_a, workload.pair_b)
     lanes = list(_native.DEFAULT_LANES)
     report = {}
-    for c in range(1, 8):
+    for c in range(0, 8):
         idx = np.nonzero(cls == c)[0]
         if len(idx) < 256:
             continue

@@ -82,6 +82,16 @@ __device__ __forceinline__ int readlane_i32(int v, int lane) {
     return __builtin_amdgcn_readlane(v, lane);
 }
 
+// For values the caller knows to be equal in every active lane: moves them to
+// scalar registers (readfirstlane), so loads they address become scalar loads.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double wave_uniform(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // ------------------------------------------------------------- colliders
 
 // Collider record as laid out in the scene (see d3d.h).
@@ -149,7 +159,7 @@ __device__ __forceinline__ bool load_checked(const d3d_scene &s, int k, Shape &h
             (cn == 0.0) ||
             (s.clusters != nullptr && c0 >= 0.0 && c0 == floor(c0) &&
              cn == (double)((h.nv + D3D_CLUSTER_SIZE - 1) / D3D_CLUSTER_SIZE) &&
-             c0 + cn <= (double)s.n_clusters);
+             c0 + cn <= (double)s.n_clusters && h.voff % D3D_CLUSTER_SIZE == 0);
         if (!ok32 || !ok64 || !okc) {
             h = empty_shape();
             return false;
@@ -240,8 +250,9 @@ __device__ __forceinline__ void top2(float sc, int j, float &m1, float &m2, int 
 // D3D_CLUSTER_SIZE vertices, each with a sphere (c, r) such that
 // fp32(c . d) + r * |l| (one FMA chain, |l| rounded up) is at least the
 // exact score of every vertex of the cluster.  The sweep scans the cluster
-// with the largest bound first, then only the clusters whose bound is not
-// below m1 - 2.5E: a skipped vertex has exact score below m1 - 2.5E, hence
+// with the largest bound first (cooperatively), then every other cluster whose
+// bound is not below m1 - 2.5E, each by one lane (few dependent memory
+// rounds per sweep): a skipped vertex has exact score below m1 - 2.5E, hence
 // below the exact score of the winner (>= m1 - E), so the certificate keeps
 // its meaning.  The result equals the full sweep's bit for bit.
 template <int G>
@@ -298,58 +309,38 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
             bk = take ? ok : bk;
         }
         const int ks = bk < nc ? bk : 0;
-        // scan of one cluster by the group: lane sub takes slots sub, sub+G, ...
-        auto scan1 = [&](int j0) {
+        // the best cluster, scanned by the group (lane sub takes slots sub, sub+G, ...)
+        {
+            const int j0 = ks * K;
             const int cnt = min(K, nv - j0);
 #pragma unroll
             for (int o = sub % K; o < K; o += (G < K ? G : K))
-                if (o < cnt) top2(score(j0 + o), j0 + o, m1, m2, i1);
-        };
-        if (G <= K || sub < K) scan1(ks * K);
+                if ((G <= K || sub < K) && o < cnt) top2(score(j0 + o), j0 + o, m1, m2, i1);
+        }
         float thr = __double2float_rd((double)group_maxf<G>(m1) - 2.5 * E);
-        const int lane = lane_id();
-        for (int base = 0; base < nc; base += G) {
-            const int k = base + sub;
-            bool surv = false;
-            if (k < nc && k != ks) surv = !(bound(k) < thr);
-            unsigned long long m = (G == 1) ? (unsigned long long)surv : group_bits<G>(__ballot(surv), lane);
-            if (G <= K) {
-                // two clusters per step when a lane holds few slots of each
-                constexpr bool kTwo = (K / (G < K ? G : K)) <= 4;
-                while (m) {
-                    const int j0 = (base + __ffsll((long long)m) - 1) * K;
-                    m &= m - 1;
-                    if (kTwo && m) {
-                        const int j1 = (base + __ffsll((long long)m) - 1) * K;
-                        m &= m - 1;
-                        const int c0 = min(K, nv - j0), c1 = min(K, nv - j1);
-#pragma unroll
-                        for (int o = sub; o < K; o += (G < K ? G : K)) {
-                            const float s0 = o < c0 ? score(j0 + o) : -INFINITY;
-                            const float s1 = o < c1 ? score(j1 + o) : -INFINITY;
-                            if (o < c0) top2(s0, j0 + o, m1, m2, i1);
-                            if (o < c1) top2(s1, j1 + o, m1, m2, i1);
-                        }
-                    } else {
-                        scan1(j0);
-                    }
+        // every other cluster whose bound reaches the threshold is scanned whole
+        // by the lane that owns it (16-byte aligned, four vertices per load);
+        // the threshold only rises, with the lane's own running maximum
+        for (int k = sub; k < nc; k += G) {
+            if (k == ks || bound(k) < thr) continue;
+            const int j0 = k * K;
+            if (j0 + K <= nv) {
+                const float4 *__restrict__ qx = reinterpret_cast<const float4 *>(px + j0);
+                const float4 *__restrict__ qy = reinterpret_cast<const float4 *>(py + j0);
+                const float4 *__restrict__ qz = reinterpret_cast<const float4 *>(pz + j0);
+#pragma unroll 1
+                for (int c = 0; c < K / 4; ++c) {
+                    const float4 x = qx[c], y = qy[c], z = qz[c];
+                    const int j = j0 + 4 * c;
+                    top2(__builtin_fmaf(z.x, dz, __builtin_fmaf(y.x, dy, x.x * dx)), j, m1, m2, i1);
+                    top2(__builtin_fmaf(z.y, dz, __builtin_fmaf(y.y, dy, x.y * dx)), j + 1, m1, m2, i1);
+                    top2(__builtin_fmaf(z.z, dz, __builtin_fmaf(y.z, dy, x.z * dx)), j + 2, m1, m2, i1);
+                    top2(__builtin_fmaf(z.w, dz, __builtin_fmaf(y.w, dy, x.w * dx)), j + 3, m1, m2, i1);
                 }
             } else {
-                // G / K clusters per step, one per K-lane subgroup
-                constexpr int P = G > K ? G / K : 1;
-                const int q = sub / K, r = sub % K;
-                while (m) {
-                    unsigned long long mm = m;
-                    for (int t = 0; t < q; ++t) mm &= mm - 1;
-                    if (mm) {
-                        const int j = (base + __ffsll((long long)mm) - 1) * K + r;
-                        if (j < nv) top2(score(j), j, m1, m2, i1);
-                    }
-#pragma unroll
-                    for (int t = 0; t < P; ++t) m &= m - 1;
-                }
+                for (int j = j0; j < nv; ++j) top2(score(j), j, m1, m2, i1);
             }
-            if (base + G < nc) thr = __double2float_rd((double)group_maxf<G>(m1) - 2.5 * E);
+            thr = fmaxf(thr, __double2float_rd((double)m1 - 2.5 * E));
         }
     }
 #pragma unroll
@@ -485,10 +476,10 @@ __device__ __forceinline__ v3 apply_margin(const Shape &h, v3 p, v3 d) {
 }
 
 // Supports of record ia along d and of record ib along -d (the two halves of
-// a Minkowski-difference support), computed one collider after the other so
-// that a single copy of the sweep is compiled.  The G lanes of a group call it
+// a Minkowski-difference support), with a single copy of the sweep compiled.  The G lanes of a group call it
 // together with the same arguments; records are reloaded by index and must
-// have been validated (load_checked) by the caller.
+// have been validated (load_checked) by the caller.  With G == 64 the
+// arguments are wave-uniform and are held in scalar registers.
 struct SupportPair {
     v3 a, b;
 };
@@ -497,15 +488,17 @@ template <int G, bool kHull>
 __device__ __forceinline__ SupportPair support_pair(const d3d_scene &s, int ia, int ib, v3 d,
                                                     int sub) {
     v3 a = mk(0.0, 0.0, 0.0), b = a;
+    if (G == kWave) d = mk(wave_uniform(d.x), wave_uniform(d.y), wave_uniform(d.z));
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
-        const Shape h = load_shape(s, opaque(k == 0 ? ia : ib));
+        const int idx = k == 0 ? ia : ib;
+        const Shape h = load_shape(s, G == kWave ? wave_uniform(idx) : opaque(idx));
         const v3 dir = vsel(k == 0, d, -d);
         v3 out;
         if (kHull && h.type == D3D_HULL) {
             const v3 l = to_local(h, dir);
-            const int idx = hull_argmax<G>(s, hull_ref(h), l, sub);
-            out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));
+            const int hi = hull_argmax<G>(s, hull_ref(h), l, sub);
+            out = to_world(h, hull_vertex(s, h.voff, h.voff64, hi));
         } else {
             out = support_analytic(h, dir);
         }

@@ -31,8 +31,23 @@ struct EpaLds {
     double bx[kEpaMaxVerts], by[kEpaMaxVerts], bz[kEpaMaxVerts];
     double fnx[kEpaMaxFaces], fny[kEpaMaxFaces], fnz[kEpaMaxFaces], fd[kEpaMaxFaces];
     int fv0[kEpaMaxFaces], fv1[kEpaMaxFaces], fv2[kEpaMaxFaces];
-    int rv0[kEpaMaxFaces], rv1[kEpaMaxFaces], rv2[kEpaMaxFaces];
+    // directed edges (a -> b, bit a * kEpaMaxVerts + b) of the faces removed in
+    // the current iteration; all zero between iterations
+    unsigned eb[(kEpaMaxVerts * kEpaMaxVerts + 31) / 32];
 };
+
+__device__ __forceinline__ void edge_set(EpaLds &L, int a, int b) {
+    const int bit = a * kEpaMaxVerts + b;
+    atomicOr(&L.eb[bit >> 5], 1u << (bit & 31));
+}
+__device__ __forceinline__ void edge_clear(EpaLds &L, int a, int b) {
+    const int bit = a * kEpaMaxVerts + b;
+    atomicAnd(&L.eb[bit >> 5], ~(1u << (bit & 31)));
+}
+__device__ __forceinline__ bool edge_has(const EpaLds &L, int a, int b) {
+    const int bit = a * kEpaMaxVerts + b;
+    return (L.eb[bit >> 5] >> (bit & 31)) & 1u;
+}
 
 __device__ __forceinline__ v3 lds_w(const EpaLds &L, int v) { return mk(L.wx[v], L.wy[v], L.wz[v]); }
 
@@ -58,91 +73,87 @@ __device__ __forceinline__ bool face_normal(v3 w0, v3 w1, v3 w2, v3 &n, double &
     return true;
 }
 
-struct SeedVerts {
-    v3 W[4], A[4], B[4];
-};
+// Polytope vertex row r of the LDS arrays: Minkowski point w and the support
+// points on A and B.  Every lane of the wavefront computes the same values and
+// writes them to the same addresses, so rows need no broadcast.
+__device__ __forceinline__ v3 row_w(const EpaLds &L, int r) { return mk(L.wx[r], L.wy[r], L.wz[r]); }
+__device__ __forceinline__ v3 row_a(const EpaLds &L, int r) { return mk(L.ax[r], L.ay[r], L.az[r]); }
+__device__ __forceinline__ v3 row_b(const EpaLds &L, int r) { return mk(L.bx[r], L.by[r], L.bz[r]); }
+__device__ __forceinline__ void put_row(EpaLds &L, int r, v3 w, v3 a, v3 b) {
+    L.wx[r] = w.x; L.wy[r] = w.y; L.wz[r] = w.z;
+    L.ax[r] = a.x; L.ay[r] = a.y; L.az[r] = a.z;
+    L.bx[r] = b.x; L.by[r] = b.y; L.bz[r] = b.z;
+}
 
-// Seed tetrahedron containing the origin from a GJK simplex of n points
-// (uniform across the wavefront).  Returns false for a degenerate seed.
-__device__ bool epa_seed(const d3d_scene &s, int ia, int ib, SeedVerts &V, int n) {
+// Seed tetrahedron containing the origin from a GJK simplex of n points held
+// in rows 0..n-1 (uniform across the wavefront); completes rows n..3 with
+// extra supports: n == 1, the six axis directions, keeping the point farthest
+// from row 0; n == 2, four directions perpendicular to the segment, keeping
+// the point farthest from its line; n == 3, both triangle normals, keeping
+// the higher side.  One support call site (the loop below) keeps a single
+// copy of the sweep; candidates go straight to their LDS rows (n == 3 uses
+// row 4 as scratch).  Returns false for a degenerate seed.
+__device__ bool epa_seed(const d3d_scene &s, int ia, int ib, EpaLds &L, int n) {
     if (n < 1 || n > 4) return false;
-    if (n == 1) {
+    while (n < 4) {
+        const v3 w0 = row_w(L, 0);
+        v3 ab = mk(0.0, 0.0, 0.0), p1 = ab, p2 = ab, nrm = ab;
+        if (n == 2) {
+            ab = row_w(L, 1) - w0;
+            const double ax = fabs(ab.x), ay = fabs(ab.y), az = fabs(ab.z);
+            const bool ex = ax <= ay && ax <= az, ey = !ex && ay <= az;
+            const v3 e = mk(ex ? 1.0 : 0.0, ey ? 1.0 : 0.0, (!ex && !ey) ? 1.0 : 0.0);
+            p1 = cross(ab, e);
+            p2 = cross(ab, p1);
+        } else if (n == 3) {
+            nrm = cross(row_w(L, 1) - w0, row_w(L, 2) - w0);
+            if (!(dot(nrm, nrm) > 0.0)) return false;
+        }
+        const int nk = n == 1 ? 6 : n == 2 ? 4 : 2;
         int best = -1;
-        double bl = 0.0;
-        v3 bw = V.W[0], bpa = V.A[0], bpb = V.B[0];
-        for (int k = 0; k < 6; ++k) {
-            const double sg = (k & 1) ? -1.0 : 1.0;
-            const v3 d = mk(k < 2 ? sg : 0.0, (k >= 2 && k < 4) ? sg : 0.0, k >= 4 ? sg : 0.0);
+        double bl = 0.0, h0 = 0.0, h1 = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < nk; ++k) {
+            v3 d;
+            if (n == 1) {
+                const double sg = (k & 1) ? -1.0 : 1.0;
+                d = mk(k < 2 ? sg : 0.0, (k >= 2 && k < 4) ? sg : 0.0, k >= 4 ? sg : 0.0);
+            } else if (n == 2) {
+                d = sel4(k, p1, -p1, p2, -p2);
+            } else {
+                d = vsel(k == 0, nrm, -nrm);
+            }
             const SupportPair sp = support_pair<kWave, true>(s, ia, ib, d, lane_id());
-            const v3 pa = sp.a, pb = sp.b;
-            const v3 w = pa - pb;
-            const v3 e = w - V.W[0];
-            const double l = dot(e, e);
-            if (l > bl) {
-                bl = l;
-                best = k;
-                bw = w;
-                bpa = pa;
-                bpb = pb;
+            const v3 w = sp.a - sp.b;
+            if (n == 3) {
+                put_row(L, 3 + k, w, sp.a, sp.b);
+                if (k == 0) h0 = dot(w - w0, nrm);
+                else h1 = -dot(w - w0, nrm);
+            } else {
+                double l;
+                if (n == 1) {
+                    const v3 e = w - w0;
+                    l = dot(e, e);
+                } else {
+                    const v3 cr = cross(ab, w - w0);
+                    l = dot(cr, cr);
+                }
+                if (l > bl) {
+                    bl = l;
+                    best = k;
+                    put_row(L, n, w, sp.a, sp.b);
+                }
             }
         }
-        if (best < 0 || !(bl > kEps)) return false;
-        V.W[1] = bw;
-        V.A[1] = bpa;
-        V.B[1] = bpb;
-        n = 2;
-    }
-    if (n == 2) {
-        const v3 ab = V.W[1] - V.W[0];
-        const double ax = fabs(ab.x), ay = fabs(ab.y), az = fabs(ab.z);
-        const bool ex = ax <= ay && ax <= az, ey = !ex && ay <= az;
-        const v3 e = mk(ex ? 1.0 : 0.0, ey ? 1.0 : 0.0, (!ex && !ey) ? 1.0 : 0.0);
-        const v3 p1 = cross(ab, e);
-        const v3 p2 = cross(ab, p1);
-        int best = -1;
-        double bl = 0.0;
-        v3 bw = V.W[0], bpa = V.A[0], bpb = V.B[0];
-        for (int k = 0; k < 4; ++k) {
-            const v3 c = sel4(k, p1, -p1, p2, -p2);
-            const SupportPair sp = support_pair<kWave, true>(s, ia, ib, c, lane_id());
-            const v3 pa = sp.a, pb = sp.b;
-            const v3 w = pa - pb;
-            const v3 cr = cross(ab, w - V.W[0]);
-            const double l = dot(cr, cr);
-            if (l > bl) {
-                bl = l;
-                best = k;
-                bw = w;
-                bpa = pa;
-                bpb = pb;
-            }
-        }
-        if (best < 0 || !(bl > kEps * dot(ab, ab))) return false;
-        V.W[2] = bw;
-        V.A[2] = bpa;
-        V.B[2] = bpb;
-        n = 3;
-    }
-    if (n == 3) {
-        const v3 nrm = cross(V.W[1] - V.W[0], V.W[2] - V.W[0]);
-        if (!(dot(nrm, nrm) > 0.0)) return false;
-        const SupportPair sp = support_pair<kWave, true>(s, ia, ib, nrm, lane_id());
-        const v3 wp_pa = sp.a, wp_pb = sp.b;
-        const v3 wp = wp_pa - wp_pb;
-        const SupportPair sm = support_pair<kWave, true>(s, ia, ib, -nrm, lane_id());
-        const v3 wm_pa = sm.a, wm_pb = sm.b;
-        const v3 wm = wm_pa - wm_pb;
-        const double hp = dot(wp - V.W[0], nrm), hm = -dot(wm - V.W[0], nrm);
-        if (hp >= hm) {
-            V.W[3] = wp;
-            V.A[3] = wp_pa;
-            V.B[3] = wp_pb;
+        if (n == 1) {
+            if (best < 0 || !(bl > kEps)) return false;
+        } else if (n == 2) {
+            if (best < 0 || !(bl > kEps * dot(ab, ab))) return false;
         } else {
-            V.W[3] = wm;
-            V.A[3] = wm_pa;
-            V.B[3] = wm_pb;
+            if (!(h0 >= h1)) put_row(L, 3, row_w(L, 4), row_a(L, 4), row_b(L, 4));
+            if (!((h0 > h1 ? h0 : h1) > 0.0)) return false;
         }
-        if (!((hp > hm ? hp : hm) > 0.0)) return false;
+        ++n;
     }
     return true;
 }
@@ -150,12 +161,15 @@ __device__ bool epa_seed(const d3d_scene &s, int ia, int ib, SeedVerts &V, int n
 __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
     d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, int64_t n,
     const int64_t *__restrict__ n_dev, const int32_t *__restrict__ work,
-    const double *__restrict__ simplex, const int32_t *__restrict__ npts, int max_iter,
-    int max_faces, double eps, double *__restrict__ out, int32_t *__restrict__ status_out,
-    int32_t *__restrict__ iters_out) {
+    const int32_t *__restrict__ work_off, const double *__restrict__ simplex,
+    const int32_t *__restrict__ npts, int max_iter, int max_faces, double eps,
+    double *__restrict__ out, int32_t *__restrict__ status_out, int32_t *__restrict__ iters_out) {
     __shared__ EpaLds L;
     const int lane = threadIdx.x;
     const int64_t count = n_dev ? *n_dev : n;
+    if (work && work_off) work += *work_off;
+    for (int k = lane; k < (int)(sizeof(L.eb) / sizeof(L.eb[0])); k += kEpaBlock) L.eb[k] = 0u;
+    __syncthreads();
     for (int64_t item = blockIdx.x; item < count; item += gridDim.x) {
         const int64_t pid = work ? (int64_t)work[item] : item;
         Shape SA, SB;  // validation only; supports reload the records by index
@@ -171,33 +185,28 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             continue;
         }
 
-        // ---- seed
-        SeedVerts V;
+        // ---- seed: the GJK simplex rows (w, support on A, support on B) go
+        // straight into polytope rows 0..3
         const double *S = simplex + 36 * pid;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            V.W[r] = mk(S[3 * r], S[3 * r + 1], S[3 * r + 2]);
-            V.A[r] = mk(S[12 + 3 * r], S[13 + 3 * r], S[14 + 3 * r]);
-            V.B[r] = mk(S[24 + 3 * r], S[25 + 3 * r], S[26 + 3 * r]);
+        if (lane < 36) {
+            const int r = (lane % 12) / 3, c = lane % 3, part = lane / 12;
+            double *dst = part == 0 ? (c == 0 ? L.wx : c == 1 ? L.wy : L.wz)
+                        : part == 1 ? (c == 0 ? L.ax : c == 1 ? L.ay : L.az)
+                                    : (c == 0 ? L.bx : c == 1 ? L.by : L.bz);
+            dst[r] = S[lane];
         }
+        __syncthreads();
         int status = 0;
         const int ia = pa[pid], ib = pb[pid];
-        bool seed_ok = epa_seed(s, ia, ib, V, npts[pid]);
+        bool seed_ok = epa_seed(s, ia, ib, L, npts[pid]);
         int nf = 0;
         if (seed_ok) {
-            const v3 nrm = cross(V.W[1] - V.W[0], V.W[2] - V.W[0]);
-            if (dot(V.W[3] - V.W[0], nrm) > 0.0) {
-                v3 t = V.W[1]; V.W[1] = V.W[2]; V.W[2] = t;
-                t = V.A[1]; V.A[1] = V.A[2]; V.A[2] = t;
-                t = V.B[1]; V.B[1] = V.B[2]; V.B[2] = t;
-            }
-            if (lane < 4) {
-                const v3 w = sel4(lane, V.W[0], V.W[1], V.W[2], V.W[3]);
-                const v3 a = sel4(lane, V.A[0], V.A[1], V.A[2], V.A[3]);
-                const v3 b = sel4(lane, V.B[0], V.B[1], V.B[2], V.B[3]);
-                L.wx[lane] = w.x; L.wy[lane] = w.y; L.wz[lane] = w.z;
-                L.ax[lane] = a.x; L.ay[lane] = a.y; L.az[lane] = a.z;
-                L.bx[lane] = b.x; L.by[lane] = b.y; L.bz[lane] = b.z;
+            const v3 w0 = row_w(L, 0);
+            const v3 nrm = cross(row_w(L, 1) - w0, row_w(L, 2) - w0);
+            if (dot(row_w(L, 3) - w0, nrm) > 0.0) {
+                const v3 w1 = row_w(L, 1), a1 = row_a(L, 1), b1 = row_b(L, 1);
+                put_row(L, 1, row_w(L, 2), row_a(L, 2), row_b(L, 2));
+                put_row(L, 2, w1, a1, b1);
             }
             // seed faces {0,1,2}, {0,3,1}, {0,2,3}, {1,3,2}; a degenerate one
             // makes the seed degenerate
@@ -207,9 +216,9 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                 i0 = (lane == 3) ? 1 : 0;
                 i1 = (lane == 0) ? 1 : (lane == 1) ? 3 : (lane == 2) ? 2 : 3;
                 i2 = (lane == 0) ? 2 : (lane == 1) ? 1 : (lane == 2) ? 3 : 2;
-                const v3 w0 = sel4(i0, V.W[0], V.W[1], V.W[2], V.W[3]);
-                const v3 w1 = sel4(i1, V.W[0], V.W[1], V.W[2], V.W[3]);
-                const v3 w2 = sel4(i2, V.W[0], V.W[1], V.W[2], V.W[3]);
+                const v3 w0 = row_w(L, i0);
+                const v3 w1 = row_w(L, i1);
+                const v3 w2 = row_w(L, i2);
                 v3 nn;
                 double dd;
                 fok = face_normal(w0, w1, w2, nn, dd);
@@ -299,40 +308,36 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                 }
             }
             const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-            const unsigned long long m0 = __ballot(rem[0]), m1 = __ballot(rem[1]);
-            const int r0 = __popcll(m0);
-            const int nrem = r0 + __popcll(m1);
-            const int rrank0 = __popcll(m0 & lt), rrank1 = r0 + __popcll(m1 & lt);
-            if (rem[0]) { L.rv0[rrank0] = fv[0][0]; L.rv1[rrank0] = fv[0][1]; L.rv2[rrank0] = fv[0][2]; }
-            if (rem[1]) { L.rv0[rrank1] = fv[1][0]; L.rv1[rrank1] = fv[1][1]; L.rv2[rrank1] = fv[1][2]; }
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                if (rem[sl]) {
+                    edge_set(L, fv[sl][0], fv[sl][1]);
+                    edge_set(L, fv[sl][1], fv[sl][2]);
+                    edge_set(L, fv[sl][2], fv[sl][0]);
+                }
+            }
             __syncthreads();
-            // horizon edges of my removed faces (hz) and whether their new
-            // faces are non-degenerate (hv), in (removed order, edge 0..2)
-            // order
-            v3 hn[2][3];
-            double hd[2][3];
-            bool hz[2][3], hv[2][3];
+            // horizon edges of my removed faces (hz): no other removed face
+            // holds the reverse edge (faces have three distinct vertices, so
+            // a face never holds the reverse of its own edge); hv: the new
+            // face on the edge is non-degenerate.  Order: (removed face, edge).
+            // bit 3*sl+e of hzm: horizon edge; of hvm: its new face is valid
+            int hzm = 0, hvm = 0;
             int cnt[2] = {0, 0};
 #pragma unroll
             for (int sl = 0; sl < 2; ++sl) {
-                const int myrank = sl == 0 ? rrank0 : rrank1;
 #pragma unroll
                 for (int e = 0; e < 3; ++e) {
-                    hz[sl][e] = hv[sl][e] = false;
-                    hn[sl][e] = mk(0.0, 0.0, 0.0);
-                    hd[sl][e] = 0.0;
                     if (!rem[sl]) continue;
                     const int a = fv[sl][e], b = fv[sl][(e + 1) % 3];
-                    bool shared = false;
-                    for (int m = 0; m < nrem && !shared; ++m) {
-                        if (m == myrank) continue;
-                        const int q0 = L.rv0[m], q1 = L.rv1[m], q2 = L.rv2[m];
-                        shared = (q0 == b && q1 == a) || (q1 == b && q2 == a) || (q2 == b && q0 == a);
+                    if (edge_has(L, b, a)) continue;
+                    hzm |= 1 << (3 * sl + e);
+                    v3 nn;
+                    double dd;
+                    if (face_normal(lds_w(L, a), lds_w(L, b), w, nn, dd)) {
+                        hvm |= 1 << (3 * sl + e);
+                        ++cnt[sl];
                     }
-                    if (shared) continue;
-                    hz[sl][e] = true;
-                    hv[sl][e] = face_normal(lds_w(L, a), lds_w(L, b), w, hn[sl][e], hd[sl][e]);
-                    cnt[sl] += hv[sl][e] ? 1 : 0;
                 }
             }
             int tot0, tot1;
@@ -352,13 +357,21 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                 int before = sl == 0 ? pre0 : pre1;
 #pragma unroll
                 for (int e = 0; e < 3; ++e) {
-                    if (!hz[sl][e]) continue;
+                    if (!((hzm >> (3 * sl + e)) & 1)) continue;
                     if (nkept + before >= max_faces) cap = true;
-                    before += hv[sl][e] ? 1 : 0;
+                    before += (hvm >> (3 * sl + e)) & 1;
                 }
             }
             cap = __ballot(cap) != 0ull;
-            __syncthreads();  // all reads of the old face list are done
+            __syncthreads();  // all reads of the old face list and edge bits are done
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                if (rem[sl]) {
+                    edge_clear(L, fv[sl][0], fv[sl][1]);
+                    edge_clear(L, fv[sl][1], fv[sl][2]);
+                    edge_clear(L, fv[sl][2], fv[sl][0]);
+                }
+            }
             if (kept0) {
                 L.fv0[kpos0] = fv[0][0]; L.fv1[kpos0] = fv[0][1]; L.fv2[kpos0] = fv[0][2];
                 L.fnx[kpos0] = fn[0].x; L.fny[kpos0] = fn[0].y; L.fnz[kpos0] = fn[0].z;
@@ -374,13 +387,18 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                 int pos = nkept + (sl == 0 ? pre0 : pre1);
 #pragma unroll
                 for (int e = 0; e < 3; ++e) {
-                    if (!hv[sl][e]) continue;
+                    if (!((hvm >> (3 * sl + e)) & 1)) continue;
                     if (pos < max_faces) {
-                        L.fv0[pos] = fv[sl][e];
-                        L.fv1[pos] = fv[sl][(e + 1) % 3];
+                        // same expression as in the scan above: same bits
+                        const int a = fv[sl][e], b = fv[sl][(e + 1) % 3];
+                        v3 nn;
+                        double dd;
+                        face_normal(lds_w(L, a), lds_w(L, b), w, nn, dd);
+                        L.fv0[pos] = a;
+                        L.fv1[pos] = b;
                         L.fv2[pos] = wi;
-                        L.fnx[pos] = hn[sl][e].x; L.fny[pos] = hn[sl][e].y; L.fnz[pos] = hn[sl][e].z;
-                        L.fd[pos] = hd[sl][e];
+                        L.fnx[pos] = nn.x; L.fny[pos] = nn.y; L.fnz[pos] = nn.z;
+                        L.fd[pos] = dd;
                     }
                     ++pos;
                 }
@@ -456,9 +474,9 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
 }
 
 int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n_max,
-               const int64_t *n_dev, const int32_t *work, const double *simplex,
-               const int32_t *npts, int max_iter, int max_faces, double eps, double *out,
-               int32_t *status, int32_t *iters, hipStream_t stream) {
+               const int64_t *n_dev, const int32_t *work, const int32_t *work_off,
+               const double *simplex, const int32_t *npts, int max_iter, int max_faces,
+               double eps, double *out, int32_t *status, int32_t *iters, hipStream_t stream) {
     if (n_max <= 0) return D3D_OK;
     int64_t grid = n_max;
     if (n_dev) {
@@ -471,7 +489,8 @@ int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t
         grid = 65536;
     }
     hipLaunchKernelGGL(epa_kernel, dim3((unsigned)grid), dim3(kEpaBlock), 0, stream, s, pa, pb, n_max,
-                       n_dev, work, simplex, npts, max_iter, max_faces, eps, out, status, iters);
+                       n_dev, work, work_off, simplex, npts, max_iter, max_faces, eps, out, status,
+                       iters);
     D3D_CHECK_LAUNCH("epa_kernel");
     return D3D_OK;
 }
@@ -492,7 +511,7 @@ extern "C" int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, cons
     D3D_CHECK_ARG(max_iterations >= 1 && max_iterations <= kEpaMaxIter,
                   "max_iterations must be in [1, 64]");
     D3D_CHECK_ARG(max_faces >= 4 && max_faces <= kEpaMaxFaces, "max_faces must be in [4, 128]");
-    return launch_epa(*scene, pair_a, pair_b, n, nullptr, nullptr, simplex, n_points,
+    return launch_epa(*scene, pair_a, pair_b, n, nullptr, nullptr, nullptr, simplex, n_points,
                       (int)max_iterations, (int)max_faces, epsilon, out, status, iterations,
                       (hipStream_t)stream);
 }

@@ -171,35 +171,43 @@ __global__ __launch_bounds__(kQBlock) void queue_scatter_kernel(
 // VGPRs leaves the registers to the simplex solve.  Structure of arrays over
 // the block's threads: component c (slot * 6 + {Px,Py,Pz,Qx,Qy,Qz}) of thread
 // t at base[c * kQBlock + t], conflict-free.
+// P, Q rows of the simplex in LDS, structure of arrays with stride `st`
+// doubles: one slot per pair, shared by the G lanes of a group (all lanes
+// read it; the group's first lane, `wr`, writes it; LDS operations of a
+// wavefront complete in order, so the other lanes see the writes).
 struct PQ {
     double *base;
+    int st;
+    bool wr;
     __device__ __forceinline__ void put(int slot, v3 p, v3 q) const {
-        double *b = base + slot * 6 * kQBlock;
+        if (!wr) return;
+        double *b = base + slot * 6 * st;
         b[0] = p.x;
-        b[kQBlock] = p.y;
-        b[2 * kQBlock] = p.z;
-        b[3 * kQBlock] = q.x;
-        b[4 * kQBlock] = q.y;
-        b[5 * kQBlock] = q.z;
+        b[st] = p.y;
+        b[2 * st] = p.z;
+        b[3 * st] = q.x;
+        b[4 * st] = q.y;
+        b[5 * st] = q.z;
     }
     __device__ __forceinline__ v3 P(int slot) const {
-        const double *b = base + slot * 6 * kQBlock;
-        return mk(b[0], b[kQBlock], b[2 * kQBlock]);
+        const double *b = base + slot * 6 * st;
+        return mk(b[0], b[st], b[2 * st]);
     }
     __device__ __forceinline__ v3 Q(int slot) const {
-        const double *b = base + slot * 6 * kQBlock;
-        return mk(b[3 * kQBlock], b[4 * kQBlock], b[5 * kQBlock]);
+        const double *b = base + slot * 6 * st;
+        return mk(b[3 * st], b[4 * st], b[5 * st]);
     }
     __device__ __forceinline__ void zero() const {
+        if (!wr) return;
 #pragma unroll
-        for (int c = 0; c < 24; ++c) base[c * kQBlock] = 0.0;
+        for (int c = 0; c < 24; ++c) base[c * st] = 0.0;
     }
     __device__ __forceinline__ void move(int from, int to) const {
-        if (from == to) return;
-        const double *a = base + from * 6 * kQBlock;
-        double *b = base + to * 6 * kQBlock;
+        if (from == to || !wr) return;
+        const double *a = base + from * 6 * st;
+        double *b = base + to * 6 * st;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) b[c * kQBlock] = a[c * kQBlock];
+        for (int c = 0; c < 6; ++c) b[c * st] = a[c * st];
     }
     // applies a compaction map (sources are >= destinations, so moving
     // rows in increasing order reads every row before it is overwritten)
@@ -228,11 +236,23 @@ __device__ __forceinline__ void gjk_init(GjkState &g, const PQ &pq) {
     g.state = -1;
 }
 
+// Y rows are P - Q rows (w = p - q when a row is appended; compaction moves
+// Y, P, Q rows alike; zero rows stay zero), so Y is rebuilt from the LDS rows
+// at the start of each step instead of being held in registers across the
+// supports.  Row for row the values are the ones the reference keeps in Y.
+__device__ __forceinline__ void load_y(Simplex &S, const PQ &pq) {
+    S.Y0 = pq.P(0) - pq.Q(0);
+    S.Y1 = pq.P(1) - pq.Q(1);
+    S.Y2 = pq.P(2) - pq.Q(2);
+    S.Y3 = pq.P(3) - pq.Q(3);
+}
+
 // One pass of _distance_loop (_gjk_jolt.py:224-288) given the supports p, q.
 // Returns false once the pair has terminated (g.state set).
 __device__ __forceinline__ bool gjk_distance_step(GjkState &g, const PQ &pq, v3 p, v3 q,
                                                   double maxd2, double tol2) {
     ++g.it;
+    load_y(g.S, pq);
     const v3 w = p - q;
     const double dw = dot(g.d, w);
     if (dw < 0.0 && dw * dw > g.vv * maxd2) {
@@ -276,6 +296,7 @@ struct GjkOut {
     int32_t *status, *iters;
     double *simplex;
     int32_t *npts;
+    bool simplex_hits_only;  // write simplex / npts only for intersecting pairs
 };
 
 // calculate_closest_points (_gjk_jolt.py:667-687)
@@ -305,10 +326,13 @@ __device__ __forceinline__ void closest_points(const Simplex &S, const PQ &pq, v
 __device__ __forceinline__ void gjk_write(const GjkState &g, const PQ &pq, int64_t pid,
                                           double sanity, const GjkOut &o) {
     if (o.iters) o.iters[pid] = g.it;
-    if (o.npts) o.npts[pid] = g.S.n;
-    if (o.simplex) {
+    Simplex S = g.S;
+    load_y(S, pq);
+    const bool want_s = !o.simplex_hits_only || g.state == D3D_INTERSECT;
+    if (o.npts && want_s) o.npts[pid] = g.S.n;
+    if (o.simplex && want_s) {
         double *w = o.simplex + 36 * pid;
-        const v3 y[4] = {g.S.Y0, g.S.Y1, g.S.Y2, g.S.Y3};
+        const v3 y[4] = {S.Y0, S.Y1, S.Y2, S.Y3};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const v3 P = pq.P(k), Q = pq.Q(k);
@@ -331,7 +355,7 @@ __device__ __forceinline__ void gjk_write(const GjkState &g, const PQ &pq, int64
         return;
     }
     v3 a, b;
-    closest_points(g.S, pq, a, b);
+    closest_points(S, pq, a, b);
     int st = g.state;
     if (!(fabs(dot(g.d, g.d) - g.vv) < sanity)) st |= D3D_SANITY_FAILED;
     const double dd = sqrt(g.vv);
@@ -365,8 +389,10 @@ __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 }
 
+// Occupancy 3 (<= 168 VGPRs): measured faster than occupancy 2 without the
+// register cap, at the cost of a few spilled registers in some instances.
 template <int G, bool kHull>
-__global__ __launch_bounds__(kQBlock) void gjk_queue_kernel(
+__global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) void gjk_queue_kernel(
     d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
     const int32_t *__restrict__ order, const int32_t *__restrict__ cls_begin,
     const int32_t *__restrict__ cls_count, int32_t *__restrict__ cls_cursor, int cls,
@@ -377,22 +403,38 @@ __global__ __launch_bounds__(kQBlock) void gjk_queue_kernel(
     const int sub = lane & (G - 1);
     const int leader = lane - sub;
     const double tol2 = prm.tolerance * prm.tolerance;
-    __shared__ double pq_lds[24 * kQBlock];
-    const PQ pq{pq_lds + threadIdx.x};
+    constexpr int kSlots = kQBlock / G;  // pairs per block
+    __shared__ double pq_lds[24 * kSlots];
+    const PQ pq{pq_lds + threadIdx.x / G, kSlots, (threadIdx.x % G) == 0};
     GjkState g;
     gjk_init(g, pq);
     int ia = 0, ib = 0;
     int64_t pid = 0;
     bool have = false, exhausted = false;
+    constexpr int kChunk = 2 * (kWave / G);
+    int cb = 0, ce = 0;  // the wave's unclaimed items [cb, ce) of the class
     while (true) {
         const bool need = !have && !exhausted;
         const unsigned long long m = __ballot(need && sub == 0);
         if (m != 0ull) {
-            const int first = __ffsll((long long)m) - 1;
-            int base = 0;
-            if (lane == first) base = atomicAdd(&cls_cursor[cls], __popcll(m));
-            base = readlane_i32(base, first);
+            // groups take items from the wave's chunk of the class range; one
+            // atomic fetches the next chunk of kChunk items when it runs out
+            const int c = __popcll(m);
             const int rank = __shfl(__popcll(m & lanemask_lt(lane)), leader);
+            const int avail = ce - cb;
+            int nb = 0;
+            if (avail < c) {
+                const int first = __ffsll((long long)m) - 1;
+                if (lane == first) nb = atomicAdd(&cls_cursor[cls], kChunk);
+                nb = readlane_i32(nb, first);
+            }
+            const int base = rank < avail ? cb : nb - avail;
+            if (avail < c) {
+                cb = nb + (c - avail);
+                ce = nb + kChunk;
+            } else {
+                cb += c;
+            }
             if (need) {
                 const int idx = base + rank;
                 if (idx < count) {
@@ -493,45 +535,16 @@ static void launch_class(int cls, int64_t n, const d3d_scene &s, const int32_t *
                        s, pa, pb, w.order, w.cls_begin, w.cls_count, w.cls_cursor, cls, prm, o);
 }
 
-static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n,
-                      const d3d_gjk_params &prm, const GjkOut &o, void *ws, hipStream_t stream) {
-    const QueueWs w = ws_view(ws, n);
-    const int hdr = (int)ws_header_ints();
-    hipLaunchKernelGGL(queue_clear_kernel, dim3(4), dim3(kQBlock), 0, stream, (int32_t *)ws, hdr);
-    D3D_CHECK_LAUNCH("queue_clear_kernel");
-    int64_t g = (n + kQBlock - 1) / kQBlock;
-    if (g > 2048) g = 2048;
-    hipLaunchKernelGGL(queue_hist_kernel, dim3((unsigned)g), dim3(kQBlock), 0, stream, s, pa, pb, n,
-                       w.hist);
-    D3D_CHECK_LAUNCH("queue_hist_kernel");
-    hipLaunchKernelGGL(queue_scan_kernel, dim3(1), dim3(kQBlock), 0, stream, w.hist, w.key_off,
-                       w.cls_begin, w.cls_count);
-    D3D_CHECK_LAUNCH("queue_scan_kernel");
-    hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)g), dim3(kQBlock), 0, stream, s, pa, pb,
-                       n, w.key_off, w.key_cur, w.order);
-    D3D_CHECK_LAUNCH("queue_scatter_kernel");
-    // big-hull classes first so their long-running groups start early
-    for (int cls = kNumClasses - 1; cls >= 0; --cls) {
-        int G = prm.lanes_per_pair[cls] > 0 ? prm.lanes_per_pair[cls] : kDefaultLanes[cls];
-        if (cls == 0) {
-            launch_class<1, false>(cls, n, s, pa, pb, w, prm, o, stream);
-        } else {
-            switch (G) {
-            case 1: launch_class<1, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
-            case 2: launch_class<2, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
-            case 4: launch_class<4, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
-            case 8: launch_class<8, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
-            case 16: launch_class<16, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
-            case 32: launch_class<32, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
-            default: launch_class<64, true>(cls, n, s, pa, pb, w, prm, o, stream); break;
-            }
-        }
-        D3D_CHECK_LAUNCH("gjk_queue_kernel");
-    }
-    return D3D_OK;
-}
-
-// ------------------------------------------------- GJK -> EPA pipeline
+// EPA behind the GJK classes (after the join, on the caller's stream): the
+// intersecting pairs are compacted into `work` and EPA runs over them.
+struct EpaHook {
+    int32_t *work;
+    unsigned long long *count;
+    int max_iter, max_faces;
+    double eps;
+    double *out;
+    int32_t *status;
+};
 
 __global__ void zero_u64_kernel(unsigned long long *c) { *c = 0ull; }
 
@@ -560,6 +573,106 @@ __global__ __launch_bounds__(kQBlock) void epa_compact_kernel(const int32_t *__r
         epa_status[i] = -1;
     }
 }
+
+// The class kernels are persistent and size their grids to the device, so
+// they are issued on four streams (the caller's and three side streams, all
+// at default priority) to let one class's tail overlap the next class's
+// work.  A fork event orders the side streams after the sort kernels and
+// join events order the caller's stream after every class; the sequence
+// records the same way into a HIP graph when the caller's stream is being
+// captured.
+struct SideStreams {
+    hipStream_t s[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[3] = {nullptr, nullptr, nullptr};
+};
+static std::mutex g_side_mu;
+static std::map<int, SideStreams> g_side;
+
+static int side_streams_locked(SideStreams **out) {
+    int dev = 0;
+    D3D_CHECK_HIP(hipGetDevice(&dev));
+    SideStreams &S = g_side[dev];
+    if (!S.fork) {
+        for (auto &x : S.s) D3D_CHECK_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        for (auto &e : S.join) D3D_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        D3D_CHECK_HIP(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+    }
+    *out = &S;
+    return D3D_OK;
+}
+
+int prepare_side_streams() {
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    SideStreams *S = nullptr;
+    return side_streams_locked(&S);
+}
+
+static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n,
+                      const d3d_gjk_params &prm, const GjkOut &o, void *ws, hipStream_t stream,
+                      const EpaHook *epa) {
+    const QueueWs w = ws_view(ws, n);
+    const int hdr = (int)ws_header_ints();
+    hipLaunchKernelGGL(queue_clear_kernel, dim3(4), dim3(kQBlock), 0, stream, (int32_t *)ws, hdr);
+    D3D_CHECK_LAUNCH("queue_clear_kernel");
+    int64_t g = (n + kQBlock - 1) / kQBlock;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(queue_hist_kernel, dim3((unsigned)g), dim3(kQBlock), 0, stream, s, pa, pb, n,
+                       w.hist);
+    D3D_CHECK_LAUNCH("queue_hist_kernel");
+    hipLaunchKernelGGL(queue_scan_kernel, dim3(1), dim3(kQBlock), 0, stream, w.hist, w.key_off,
+                       w.cls_begin, w.cls_count);
+    D3D_CHECK_LAUNCH("queue_scan_kernel");
+    hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)g), dim3(kQBlock), 0, stream, s, pa, pb,
+                       n, w.key_off, w.key_cur, w.order);
+    D3D_CHECK_LAUNCH("queue_scatter_kernel");
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    SideStreams *S = nullptr;
+    if (int rc = side_streams_locked(&S)) return rc;
+    D3D_CHECK_HIP(hipEventRecord(S->fork, stream));
+    for (auto &x : S->s) D3D_CHECK_HIP(hipStreamWaitEvent(x, S->fork, 0));
+    const hipStream_t q[4] = {stream, S->s[0], S->s[1], S->s[2]};
+    // big-hull classes first so their long-running groups start early
+    for (int cls = kNumClasses - 1, k = 0; cls >= 0; --cls, ++k) {
+        const hipStream_t st = q[k % 4];
+        int G = prm.lanes_per_pair[cls] > 0 ? prm.lanes_per_pair[cls] : kDefaultLanes[cls];
+        if (cls == 0) {
+            switch (G) {
+            case 1: launch_class<1, false>(cls, n, s, pa, pb, w, prm, o, st); break;
+            case 2: launch_class<2, false>(cls, n, s, pa, pb, w, prm, o, st); break;
+            default: launch_class<4, false>(cls, n, s, pa, pb, w, prm, o, st); break;
+            }
+        } else {
+            switch (G) {
+            case 1: launch_class<1, true>(cls, n, s, pa, pb, w, prm, o, st); break;
+            case 2: launch_class<2, true>(cls, n, s, pa, pb, w, prm, o, st); break;
+            case 4: launch_class<4, true>(cls, n, s, pa, pb, w, prm, o, st); break;
+            case 8: launch_class<8, true>(cls, n, s, pa, pb, w, prm, o, st); break;
+            case 16: launch_class<16, true>(cls, n, s, pa, pb, w, prm, o, st); break;
+            case 32: launch_class<32, true>(cls, n, s, pa, pb, w, prm, o, st); break;
+            default: launch_class<64, true>(cls, n, s, pa, pb, w, prm, o, st); break;
+            }
+        }
+        D3D_CHECK_LAUNCH("gjk_queue_kernel");
+    }
+    for (int k = 0; k < 3; ++k) {
+        D3D_CHECK_HIP(hipEventRecord(S->join[k], S->s[k]));
+        D3D_CHECK_HIP(hipStreamWaitEvent(stream, S->join[k], 0));
+    }
+    if (epa) {
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, epa->count);
+        D3D_CHECK_LAUNCH("zero_u64_kernel");
+        hipLaunchKernelGGL(epa_compact_kernel, dim3(grid_for(n, kQBlock)), dim3(kQBlock), 0, stream,
+                           o.status, n, epa->work, epa->count, epa->out, epa->status);
+        D3D_CHECK_LAUNCH("epa_compact_kernel");
+        return launch_epa(s, pa, pb, n, (const int64_t *)epa->count, epa->work, nullptr, o.simplex,
+                          o.npts, epa->max_iter, epa->max_faces, epa->eps, epa->out, epa->status,
+                          nullptr, stream);
+    }
+    return D3D_OK;
+}
+
+// ------------------------------------------------- GJK -> EPA pipeline
 
 // Per-pair collider instances: record 2i is scene record a[i] posed by
 // pose_a[i] (R row-major, t: 12 doubles), record 2i+1 is b[i] posed by
@@ -622,6 +735,7 @@ static int gjk_distance_impl(const d3d_scene *scene, const int32_t *pair_a, cons
                              int64_t n, const d3d_gjk_params *params, double *dist, double *cp_a,
                              double *cp_b, int32_t *status, int32_t *iterations, double *simplex,
                              int32_t *n_points, void *workspace, int64_t workspace_bytes,
+                             const EpaHook *epa,
                              void *stream) {
     clear_error();
     if (int rc = check_scene(scene)) return rc;
@@ -652,8 +766,8 @@ static int gjk_distance_impl(const d3d_scene *scene, const int32_t *pair_a, cons
     } else {
         if (int rc = workspace_for(stream, n, &ws)) return rc;
     }
-    GjkOut o{dist, cp_a, cp_b, status, iterations, simplex, n_points};
-    return launch_all(*scene, pair_a, pair_b, n, prm, o, ws, (hipStream_t)stream);
+    GjkOut o{dist, cp_a, cp_b, status, iterations, simplex, n_points, epa != nullptr};
+    return launch_all(*scene, pair_a, pair_b, n, prm, o, ws, (hipStream_t)stream, epa);
 }
 
 extern "C" int d3d_gjk_distance_batch(const d3d_scene *scene, const int32_t *pair_a,
@@ -662,7 +776,7 @@ extern "C" int d3d_gjk_distance_batch(const d3d_scene *scene, const int32_t *pai
                                       double *cp_b, int32_t *status, int32_t *iterations,
                                       double *simplex, int32_t *n_points, void *stream) {
     return gjk_distance_impl(scene, pair_a, pair_b, n, params, dist, cp_a, cp_b, status, iterations,
-                             simplex, n_points, nullptr, 0, stream);
+                             simplex, n_points, nullptr, 0, nullptr, stream);
 }
 
 extern "C" int d3d_gjk_distance_batch_ws(const d3d_scene *scene, const int32_t *pair_a,
@@ -674,7 +788,7 @@ extern "C" int d3d_gjk_distance_batch_ws(const d3d_scene *scene, const int32_t *
     clear_error();
     D3D_CHECK_ARG(workspace != nullptr, "workspace is NULL");
     return gjk_distance_impl(scene, pair_a, pair_b, n, params, dist, cp_a, cp_b, status, iterations,
-                             simplex, n_points, workspace, workspace_bytes, stream);
+                             simplex, n_points, workspace, workspace_bytes, nullptr, stream);
 }
 
 // Workspace of the fused pipeline: GJK queue | simplex (n*36 f64) |
@@ -709,18 +823,10 @@ extern "C" int d3d_gjk_epa_batch_ws(const d3d_scene *scene, const int32_t *pair_
     int32_t *work = (int32_t *)(w + gq + a256(n * 36 * 8) + a256(n * 4));
     unsigned long long *count =
         (unsigned long long *)(w + gq + a256(n * 36 * 8) + a256(n * 4) + a256(n * 4));
-    if (int rc = gjk_distance_impl(scene, pair_a, pair_b, n, params, dist, cp_a, cp_b, status,
-                                   iterations, simplex, npts, workspace, gq, stream))
-        return rc;
-    hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, st, count);
-    D3D_CHECK_LAUNCH("zero_u64_kernel");
-    hipLaunchKernelGGL(epa_compact_kernel, dim3(grid_for(n, kQBlock)), dim3(kQBlock), 0, st, status, n,
-                       work, count, epa_out, epa_status);
-    D3D_CHECK_LAUNCH("epa_compact_kernel");
-    return launch_epa(*scene, pair_a, pair_b, n, (const int64_t *)count, work, simplex, npts,
-                      (int)epa_max_iterations, (int)epa_max_faces, epa_epsilon, epa_out, epa_status,
-                      nullptr, st);
+    const EpaHook hook{work, count, (int)epa_max_iterations, (int)epa_max_faces, epa_epsilon,
+                       epa_out, epa_status};
+    return gjk_distance_impl(scene, pair_a, pair_b, n, params, dist, cp_a, cp_b, status, iterations,
+                             simplex, npts, workspace, gq, &hook, stream);
 }
 
 extern "C" int d3d_instance_pairs(const d3d_scene *scene, const int32_t *pair_a,

@@ -40,6 +40,10 @@ void clear_error();
         }                                                                            \
     } while (0)
 
+// Creates the per-device side streams of the GJK class launches (called by
+// d3d_init so that no stream is created while a caller captures a graph).
+int prepare_side_streams();
+
 inline int check_scene(const d3d_scene *s) {
     if (!s) {
         set_error("scene is NULL");
@@ -68,9 +72,11 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 
 // EPA launcher shared by d3d_epa_batch and the fused GJK+EPA pipeline: with
 // n_dev/work set, a persistent grid takes pair indices work[0 .. *n_dev).
+// EPA over the pairs work[*work_off + i], i < *n_dev (device values; work_off
+// may be NULL), or over pairs 0..n_max-1 when work is NULL.
 int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n_max,
-               const int64_t *n_dev, const int32_t *work, const double *simplex,
-               const int32_t *npts, int max_iter, int max_faces, double eps, double *out,
-               int32_t *status, int32_t *iters, hipStream_t stream);
+               const int64_t *n_dev, const int32_t *work, const int32_t *work_off,
+               const double *simplex, const int32_t *npts, int max_iter, int max_faces,
+               double eps, double *out, int32_t *status, int32_t *iters, hipStream_t stream);
 
 }  // namespace d3d

@@ -59,7 +59,9 @@ extern "C" {
  *   shape_f64[k*16 + 12..14] type parameters:
  *       HULL: p0 = max_i(|x_i|+|y_i|+|z_i|) over the local vertices,
  *             p1 = index of the hull's first cluster record, p2 = number of
- *             clusters (0: unclustered; otherwise ceil(n/D3D_CLUSTER_SIZE))
+ *             clusters (0: unclustered; otherwise ceil(n/D3D_CLUSTER_SIZE),
+ *             and the hull's FP32-pool offset is a multiple of
+ *             D3D_CLUSTER_SIZE)
  *       SPHERE: radius; CAPSULE: radius, height; CYLINDER: radius, length;
  *       CONE: radius, height; DISK: radius (R columns = plane basis, normal);
  *       ELLIPSE: radius0, radius1 (R columns 0,1 = axes); BOX: size[3];
@@ -108,7 +110,9 @@ typedef struct d3d_scene {
  *   class 0: n = 0, 1: n <= 32, 2: <= 128, 3: <= 512, 4: <= 1024,
  *   5: <= 2048, 6: <= 4096, 7: > 4096.
  * lanes_per_pair[c] lanes (a power of two <= 64; 0 = built-in default
- * 1,1,1,1,2,4,8,16) work on each pair of class c; class 0 always uses 1. */
+ * 1,1,1,1,2,4,8,16) work on each pair of class c (class 0: 1, 2 or 4; larger
+ * values use 4).  With 2..32 lanes the two halves of a group evaluate the
+ * supports on A and on B concurrently. */
 typedef struct d3d_gjk_params {
     double tolerance;            /* 1e-10 */
     double max_distance_squared; /* 1e5 */

@@ -236,3 +236,19 @@ def test_dropin_support_and_aabb(gpu):  # test_colliders.py:35-93
     assert_array_almost_equal(box.aabb(), [[-0.6, 0.6]] * 3)
     assert_array_almost_equal(colliders.Sphere(np.zeros(3), 1.0).support_function(
         np.array([0.0, 0.0, 0.0])), [0, 0, 1])
+
+
+def test_cluster_layouts_and_group_sizes_agree_bitwise(gpu):
+    """Pruned sweeps over the clustered layout give the oracle's supports for
+    every lanes-per-pair value and clustering threshold (0 = plain layout)."""
+    from distance3d_amd.scene import DeviceScene
+    w = workloads.mixed_pairs(seed=12, n_pairs=12000)
+    o = oracle.gjk_distance(w.scene, w.pair_a, w.pair_b)
+    for cmin in (0, 16, 64):
+        dev = DeviceScene(w.scene, cluster_min=cmin)
+        for g in (1, 2, 4, 16, 32, 64):
+            r = kernels.gjk_distance_batch(dev, w.pair_a, w.pair_b,
+                                           lanes_per_pair=[1] + [g] * 7)
+            assert np.array_equal(_np(r.dist), o["dist"]), (cmin, g)
+            assert np.array_equal(_np(r.status), o["status"]), (cmin, g)
+            assert np.array_equal(_np(r.cp_a), o["cp_a"], equal_nan=True), (cmin, g)
