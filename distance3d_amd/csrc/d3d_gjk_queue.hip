@@ -389,43 +389,41 @@ __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 }
 
-// Occupancy 3 (<= 168 VGPRs): measured faster than occupancy 2 without the
-// register cap, at the cost of a few spilled registers in some instances.
+// One work class, G lanes per pair, by one wavefront: groups of G lanes take
+// pairs of the class (chunked claims from the class cursor) until the class is
+// exhausted and every group of the wavefront has finished its pair.  The
+// wavefront's P/Q slots are `pq_wave` (24 x 64 doubles, slot stride 64 / G).
 template <int G, bool kHull>
-__global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) void gjk_queue_kernel(
-    d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
-    const int32_t *__restrict__ order, const int32_t *__restrict__ cls_begin,
-    const int32_t *__restrict__ cls_count, int32_t *__restrict__ cls_cursor, int cls,
-    d3d_gjk_params prm, GjkOut o) {
-    const int begin = cls_begin[cls];
-    const int count = cls_count[cls];
+__device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__restrict__ pa,
+                                          const int32_t *__restrict__ pb,
+                                          const int32_t *__restrict__ order, int begin, int count,
+                                          int32_t *__restrict__ cursor, const d3d_gjk_params &prm,
+                                          const GjkOut &o, double *pq_wave) {
     const int lane = lane_id();
     const int sub = lane & (G - 1);
     const int leader = lane - sub;
     const double tol2 = prm.tolerance * prm.tolerance;
-    constexpr int kSlots = kQBlock / G;  // pairs per block
-    __shared__ double pq_lds[24 * kSlots];
-    const PQ pq{pq_lds + threadIdx.x / G, kSlots, (threadIdx.x % G) == 0};
+    const PQ pq{pq_wave + lane / G, kWave / G, sub == 0};
     GjkState g;
     gjk_init(g, pq);
     int ia = 0, ib = 0;
-    int64_t pid = 0;
+    int pid = 0;
     bool have = false, exhausted = false;
     constexpr int kChunk = 2 * (kWave / G);
-    int cb = 0, ce = 0;  // the wave's unclaimed items [cb, ce) of the class
+    int cb = 0, ce = 0;  // the wavefront's unclaimed items [cb, ce) of the class
     while (true) {
         const bool need = !have && !exhausted;
         const unsigned long long m = __ballot(need && sub == 0);
         if (m != 0ull) {
-            // groups take items from the wave's chunk of the class range; one
-            // atomic fetches the next chunk of kChunk items when it runs out
+            // groups take items from the wavefront's chunk of the class range;
+            // one atomic fetches the next chunk of kChunk items when it runs out
             const int c = __popcll(m);
             const int rank = __shfl(__popcll(m & lanemask_lt(lane)), leader);
             const int avail = ce - cb;
             int nb = 0;
             if (avail < c) {
                 const int first = __ffsll((long long)m) - 1;
-                if (lane == first) nb = atomicAdd(&cls_cursor[cls], kChunk);
+                if (lane == first) nb = atomicAdd(cursor, kChunk);
                 nb = readlane_i32(nb, first);
             }
             const int base = rank < avail ? cb : nb - avail;
@@ -479,6 +477,43 @@ __global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
     }
 }
 
+// All work classes in one persistent launch: every wavefront walks the classes
+// from the largest hulls (7) to the analytic pairs (0), switching its lanes per
+// pair at each class, so classes overlap without a launch per class.
+// lanes_per_pair holds resolved values (1..64, class 0: 1, 2 or 4).
+// Occupancy 3 (<= 168 VGPRs): measured faster than occupancy 2 without the
+// register cap, at the cost of a few spilled registers.
+__global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) void gjk_all_kernel(
+    d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb,
+    const int32_t *__restrict__ order, const int32_t *__restrict__ cls_begin,
+    const int32_t *__restrict__ cls_count, int32_t *__restrict__ cls_cursor, d3d_gjk_params prm,
+    GjkOut o) {
+    __shared__ double pq_lds[24 * kQBlock];
+    double *pq_wave = pq_lds + 24 * kWave * (threadIdx.x / kWave);
+    for (int cls = kNumClasses - 1; cls >= 0; --cls) {
+        const int count = cls_count[cls];
+        if (count == 0) continue;
+        const int begin = cls_begin[cls];
+        int32_t *cur = cls_cursor + cls;
+        const int G = prm.lanes_per_pair[cls];
+        if (cls == 0) {
+            if (G <= 1) run_class<1, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave);
+            else if (G == 2) run_class<2, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave);
+            else run_class<4, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave);
+        } else {
+            switch (G) {
+            case 1: run_class<1, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
+            case 2: run_class<2, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
+            case 4: run_class<4, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
+            case 8: run_class<8, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
+            case 16: run_class<16, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
+            case 32: run_class<32, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
+            default: run_class<64, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- host
 
 static const int kDefaultLanes[kNumClasses] = {1, 1, 1, 1, 2, 4, 8, 16};
@@ -522,19 +557,6 @@ static int cu_count() {
     return cus;
 }
 
-template <int G, bool kHull>
-static void launch_class(int cls, int64_t n, const d3d_scene &s, const int32_t *pa,
-                         const int32_t *pb, const QueueWs &w, const d3d_gjk_params &prm,
-                         const GjkOut &o, hipStream_t stream) {
-    const int64_t per_block = kQBlock / G;
-    int64_t grid = (n + per_block - 1) / per_block;
-    const int64_t cap = (int64_t)cu_count() * 4;
-    if (grid > cap) grid = cap;
-    if (grid < 1) grid = 1;
-    hipLaunchKernelGGL((gjk_queue_kernel<G, kHull>), dim3((unsigned)grid), dim3(kQBlock), 0, stream,
-                       s, pa, pb, w.order, w.cls_begin, w.cls_count, w.cls_cursor, cls, prm, o);
-}
-
 // EPA behind the GJK classes (after the join, on the caller's stream): the
 // intersecting pairs are compacted into `work` and EPA runs over them.
 struct EpaHook {
@@ -574,40 +596,6 @@ __global__ __launch_bounds__(kQBlock) void epa_compact_kernel(const int32_t *__r
     }
 }
 
-// The class kernels are persistent and size their grids to the device, so
-// they are issued on four streams (the caller's and three side streams, all
-// at default priority) to let one class's tail overlap the next class's
-// work.  A fork event orders the side streams after the sort kernels and
-// join events order the caller's stream after every class; the sequence
-// records the same way into a HIP graph when the caller's stream is being
-// captured.
-struct SideStreams {
-    hipStream_t s[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t fork = nullptr;
-    hipEvent_t join[3] = {nullptr, nullptr, nullptr};
-};
-static std::mutex g_side_mu;
-static std::map<int, SideStreams> g_side;
-
-static int side_streams_locked(SideStreams **out) {
-    int dev = 0;
-    D3D_CHECK_HIP(hipGetDevice(&dev));
-    SideStreams &S = g_side[dev];
-    if (!S.fork) {
-        for (auto &x : S.s) D3D_CHECK_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
-        for (auto &e : S.join) D3D_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        D3D_CHECK_HIP(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
-    }
-    *out = &S;
-    return D3D_OK;
-}
-
-int prepare_side_streams() {
-    std::lock_guard<std::mutex> lk(g_side_mu);
-    SideStreams *S = nullptr;
-    return side_streams_locked(&S);
-}
-
 static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n,
                       const d3d_gjk_params &prm, const GjkOut &o, void *ws, hipStream_t stream,
                       const EpaHook *epa) {
@@ -626,39 +614,19 @@ static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, 
     hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)g), dim3(kQBlock), 0, stream, s, pa, pb,
                        n, w.key_off, w.key_cur, w.order);
     D3D_CHECK_LAUNCH("queue_scatter_kernel");
-    std::lock_guard<std::mutex> lk(g_side_mu);
-    SideStreams *S = nullptr;
-    if (int rc = side_streams_locked(&S)) return rc;
-    D3D_CHECK_HIP(hipEventRecord(S->fork, stream));
-    for (auto &x : S->s) D3D_CHECK_HIP(hipStreamWaitEvent(x, S->fork, 0));
-    const hipStream_t q[4] = {stream, S->s[0], S->s[1], S->s[2]};
-    // big-hull classes first so their long-running groups start early
-    for (int cls = kNumClasses - 1, k = 0; cls >= 0; --cls, ++k) {
-        const hipStream_t st = q[k % 4];
-        int G = prm.lanes_per_pair[cls] > 0 ? prm.lanes_per_pair[cls] : kDefaultLanes[cls];
-        if (cls == 0) {
-            switch (G) {
-            case 1: launch_class<1, false>(cls, n, s, pa, pb, w, prm, o, st); break;
-            case 2: launch_class<2, false>(cls, n, s, pa, pb, w, prm, o, st); break;
-            default: launch_class<4, false>(cls, n, s, pa, pb, w, prm, o, st); break;
-            }
-        } else {
-            switch (G) {
-            case 1: launch_class<1, true>(cls, n, s, pa, pb, w, prm, o, st); break;
-            case 2: launch_class<2, true>(cls, n, s, pa, pb, w, prm, o, st); break;
-            case 4: launch_class<4, true>(cls, n, s, pa, pb, w, prm, o, st); break;
-            case 8: launch_class<8, true>(cls, n, s, pa, pb, w, prm, o, st); break;
-            case 16: launch_class<16, true>(cls, n, s, pa, pb, w, prm, o, st); break;
-            case 32: launch_class<32, true>(cls, n, s, pa, pb, w, prm, o, st); break;
-            default: launch_class<64, true>(cls, n, s, pa, pb, w, prm, o, st); break;
-            }
-        }
-        D3D_CHECK_LAUNCH("gjk_queue_kernel");
+    // one persistent launch for every class; lanes resolved on the host
+    d3d_gjk_params rp = prm;
+    for (int c = 0; c < kNumClasses; ++c) {
+        int G = prm.lanes_per_pair[c] > 0 ? prm.lanes_per_pair[c] : kDefaultLanes[c];
+        if (c == 0 && G > 4) G = 4;
+        rp.lanes_per_pair[c] = G;
     }
-    for (int k = 0; k < 3; ++k) {
-        D3D_CHECK_HIP(hipEventRecord(S->join[k], S->s[k]));
-        D3D_CHECK_HIP(hipStreamWaitEvent(stream, S->join[k], 0));
-    }
+    // resident at occupancy 3 (4 wavefronts per block); wavefronts without work
+    // leave after one claim per class
+    const int64_t grid = (int64_t)cu_count() * 3;
+    hipLaunchKernelGGL(gjk_all_kernel, dim3((unsigned)grid), dim3(kQBlock), 0, stream, s, pa, pb,
+                       w.order, w.cls_begin, w.cls_count, w.cls_cursor, rp, o);
+    D3D_CHECK_LAUNCH("gjk_all_kernel");
     if (epa) {
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, epa->count);
         D3D_CHECK_LAUNCH("zero_u64_kernel");

@@ -40,10 +40,6 @@ void clear_error();
         }                                                                            \
     } while (0)
 
-// Creates the per-device side streams of the GJK class launches (called by
-// d3d_init so that no stream is created while a caller captures a graph).
-int prepare_side_streams();
-
 inline int check_scene(const d3d_scene *s) {
     if (!s) {
         set_error("scene is NULL");

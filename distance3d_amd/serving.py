@@ -191,16 +191,22 @@ class GpuSlot:
     def done(self):
         return self.event.query()
 
-    def results(self, now):
-        dist = self.h_dist.numpy()
-        cp = self.h_cp.numpy()
-        st = self.h_status.numpy()
-        epa = self.h_epa.numpy()
+    def results(self, now, arena, start):
+        """Copies the batch's outputs into ``arena`` rows [start, start + n) (the
+        batch's requests are consecutive there) and returns per-request
+        Results that are views of the arena."""
+        n = self.n_used
+        dist, cpa, cpb, st, epa, est = arena
+        dist[start:start + n] = self.h_dist.numpy()[:n]
+        cpa[start:start + n] = self.h_cp.numpy()[0, :n]
+        cpb[start:start + n] = self.h_cp.numpy()[1, :n]
+        st[start:start + n] = self.h_status.numpy()[0, :n]
+        est[start:start + n] = self.h_status.numpy()[1, :n]
+        epa[start:start + n] = self.h_epa.numpy()[:n]
         out = []
         for r, off, k in self.batch:
-            s = slice(off, off + k)
-            out.append(Result(r.rid, dist[s].copy(), cp[0, s].copy(), cp[1, s].copy(),
-                              st[0, s].copy(), epa[s].copy(), st[1, s].copy(),
+            s = slice(start + off, start + off + k)
+            out.append(Result(r.rid, dist[s], cpa[s], cpb[s], st[s], epa[s], est[s],
                               now - r.arrival))
         return out
 
@@ -245,6 +251,16 @@ class CollisionServer:
             if r.n_pairs > self.slots[0].capacity:
                 raise ValueError("request %r has %d pairs, more than a slot holds (%d)"
                                  % (r.rid, r.n_pairs, self.slots[0].capacity))
+        # one result arena for the whole stream, rows in arrival order, touched
+        # before the clock starts so that no page fault lands in a batch
+        total = int(sum(r.n_pairs for r in reqs))
+        arena = (np.zeros(total), np.zeros((total, 3)), np.zeros((total, 3)),
+                 np.zeros(total, np.int32), np.zeros((total, 13)), np.zeros(total, np.int32))
+        row = 0
+        first_row = {}
+        for r in reqs:
+            first_row[r.rid] = row
+            row += r.n_pairs
         pending = collections.deque()
         free = collections.deque(self.slots)
         busy = collections.deque()
@@ -261,7 +277,8 @@ class CollisionServer:
                 i += 1
             while busy and busy[0].done():
                 s = busy.popleft()
-                for res in s.results(clock() - t0 if realtime else 0.0):
+                start = first_row[s.batch[0][0].rid]
+                for res in s.results(clock() - t0 if realtime else 0.0, arena, start):
                     results[res.rid] = res
                 free.append(s)
             if free and pending:

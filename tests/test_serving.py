@@ -25,7 +25,12 @@ class _FakeSlot:
 
     def load(self, requests):
         assert sum(r.n_pairs for r in requests) <= self.capacity
-        self.batch = list(requests)
+        self.batch = []
+        off = 0
+        for r in requests:
+            self.batch.append((r, off, r.n_pairs))
+            off += r.n_pairs
+        self.n_used = off
         _FakeSlot.log.append([r.rid for r in requests])
 
     def launch(self):
@@ -35,14 +40,14 @@ class _FakeSlot:
         self.polls += 1
         return self.polls > self.delay
 
-    def results(self, now):
+    def results(self, now, arena, start):
+        dist = arena[0]
         out = []
-        for r in self.batch:
-            k = r.n_pairs
-            d = (r.pair_a * 1000 + r.pair_b).astype(float)
-            out.append(serving.Result(r.rid, d, np.zeros((k, 3)), np.zeros((k, 3)),
-                                      np.zeros(k, np.int32), np.zeros((k, 13)),
-                                      np.zeros(k, np.int32), now - r.arrival))
+        for r, off, k in self.batch:
+            s = slice(start + off, start + off + k)
+            dist[s] = (r.pair_a * 1000 + r.pair_b).astype(float)
+            out.append(serving.Result(r.rid, dist[s], arena[1][s], arena[2][s], arena[3][s],
+                                      arena[4][s], arena[5][s], now - r.arrival))
         return out
 
 
