@@ -255,6 +255,36 @@ __device__ __forceinline__ void top2(float sc, int j, float &m1, float &m2, int 
 // rounds per sweep): a skipped vertex has exact score below m1 - 2.5E, hence
 // below the exact score of the winner (>= m1 - E), so the certificate keeps
 // its meaning.  The result equals the full sweep's bit for bit.
+// FP64 argmax over the hull by the G lanes of a group, ties to the smallest
+// original index (vorig): the exact fallback of the FP32 certificate.
+template <int G>
+__device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H, v3 l, int sub) {
+    const int nv = H.nv;
+    double best = -INFINITY;
+    int bi = 0x7fffffff, bo = 0x7fffffff;
+    for (int j = sub; j < nv; j += G) {
+        const v3 v = hull_vertex(s, H.voff, H.voff64, j);
+        const double val = v.x * l.x + v.y * l.y + v.z * l.z;
+        const int oj = s.vorig ? s.vorig[H.voff + j] : j;
+        if (val > best || (val == best && oj < bo)) {
+            best = val;
+            bi = j;
+            bo = oj;
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        const int oo = __shfl_xor(bo, off);
+        const bool take = (ob > best) || (ob == best && oo < bo);
+        best = take ? ob : best;
+        bi = take ? oi : bi;
+        bo = take ? oo : bo;
+    }
+    return bi == 0x7fffffff ? 0 : bi;
+}
+
 template <int G>
 __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H, v3 l, int sub) {
     constexpr int K = D3D_CLUSTER_SIZE;
@@ -355,29 +385,7 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
     }
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
 
-    double best = -INFINITY;
-    int bi = 0x7fffffff, bo = 0x7fffffff;
-    for (int j = sub; j < nv; j += G) {
-        const v3 v = hull_vertex(s, H.voff, H.voff64, j);
-        const double val = v.x * l.x + v.y * l.y + v.z * l.z;
-        const int oj = s.vorig ? s.vorig[H.voff + j] : j;
-        if (val > best || (val == best && oj < bo)) {
-            best = val;
-            bi = j;
-            bo = oj;
-        }
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        const int oo = __shfl_xor(bo, off);
-        const bool take = (ob > best) || (ob == best && oo < bo);
-        best = take ? ob : best;
-        bi = take ? oi : bi;
-        bo = take ? oo : bo;
-    }
-    return bi == 0x7fffffff ? 0 : bi;
+    return hull_rescan<G>(s, H, l, sub);
 }
 
 // Closed-form supports, FP64, in the same operation order as the reference.

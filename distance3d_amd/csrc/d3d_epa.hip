@@ -23,6 +23,9 @@ namespace d3d {
 constexpr int kEpaMaxFaces = 128;
 constexpr int kEpaMaxIter = 64;
 constexpr int kEpaMaxVerts = kEpaMaxIter + 8;
+// FP32 vertex cache: a pair whose hulls have at most this many vertices in
+// total is swept from LDS (most EPA pairs; larger ones use the global sweep)
+constexpr int kEpaCache = 672;
 constexpr int kEpaBlock = 64;
 
 struct EpaLds {
@@ -34,7 +37,73 @@ struct EpaLds {
     // directed edges (a -> b, bit a * kEpaMaxVerts + b) of the faces removed in
     // the current iteration; all zero between iterations
     unsigned eb[(kEpaMaxVerts * kEpaMaxVerts + 31) / 32];
+    // the pair's hull vertices (pool order) and the offsets of A's and B's
+    // copies (-1: not cached)
+    float cx[kEpaCache], cy[kEpaCache], cz[kEpaCache];
+    int coff[2];
 };
+
+// Argmax over a hull whose FP32 vertices are cached in LDS at `off` (pool
+// order), by the 64 lanes of the wavefront: the FP32 sweep and certificate of
+// hull_argmax, with its FP64 rescan from the pools when the certificate fails.
+__device__ __forceinline__ int lds_hull_argmax(const d3d_scene &s, const EpaLds &L, int off,
+                                               const HullRef &H, v3 l) {
+    const int lane = lane_id();
+    const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
+    const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
+    const double E = 8.0 * 5.9604644775390625e-08 * H.L * linf;
+    float m1 = -INFINITY, m2 = -INFINITY;
+    int i1 = 0x7fffffff;
+    for (int j = lane; j < H.nv; j += kWave)
+        top2(__builtin_fmaf(L.cz[off + j], dz, __builtin_fmaf(L.cy[off + j], dy, L.cx[off + j] * dx)), j,
+             m1, m2, i1);
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const float om1 = __shfl_xor(m1, o);
+        const float om2 = __shfl_xor(m2, o);
+        const int oi = __shfl_xor(i1, o);
+        m2 = fmaxf(fmaxf(m2, om2), fminf(m1, om1));
+        const bool take = (om1 > m1) || (om1 == m1 && oi < i1);
+        m1 = take ? om1 : m1;
+        i1 = take ? oi : i1;
+    }
+    if ((double)m1 - (double)m2 > 2.0 * E) return i1;
+    return hull_rescan<kWave>(s, H, l, lane);
+}
+
+// Supports of record ia along d and of record ib along -d for the EPA
+// wavefront (see support_pair), sweeping LDS-cached hulls from LDS.
+__device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, const EpaLds &L, int ia,
+                                                        int ib, v3 d) {
+    v3 a = mk(0.0, 0.0, 0.0), b = a;
+    d = mk(wave_uniform(d.x), wave_uniform(d.y), wave_uniform(d.z));
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const Shape h = load_shape(s, wave_uniform(k == 0 ? ia : ib));
+        const v3 dir = vsel(k == 0, d, -d);
+        v3 out;
+        if (h.type == D3D_HULL) {
+            const v3 l = to_local(h, dir);
+            const HullRef H = hull_ref(h);
+            const int off = L.coff[k];
+            v3 v;
+            if (off >= 0) {
+                const int idx = lds_hull_argmax(s, L, off, H, l);
+                v = h.voff64 < 0 ? mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx])
+                                 : hull_vertex(s, h.voff, h.voff64, idx);
+            } else {
+                v = hull_vertex(s, h.voff, h.voff64, hull_argmax<kWave>(s, H, l, lane_id()));
+            }
+            out = to_world(h, v);
+        } else {
+            out = support_analytic(h, dir);
+        }
+        out = apply_margin(h, out, dir);
+        a = vsel(k == 0, out, a);
+        b = vsel(k == 0, b, out);
+    }
+    return SupportPair{a, b};
+}
 
 __device__ __forceinline__ void edge_set(EpaLds &L, int a, int b) {
     const int bit = a * kEpaMaxVerts + b;
@@ -123,7 +192,7 @@ __device__ bool epa_seed(const d3d_scene &s, int ia, int ib, EpaLds &L, int n) {
             } else {
                 d = vsel(k == 0, nrm, -nrm);
             }
-            const SupportPair sp = support_pair<kWave, true>(s, ia, ib, d, lane_id());
+            const SupportPair sp = epa_support_pair(s, L, ia, ib, d);
             const v3 w = sp.a - sp.b;
             if (n == 3) {
                 put_row(L, 3 + k, w, sp.a, sp.b);
@@ -185,6 +254,28 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             continue;
         }
 
+        // ---- hull vertices into the LDS cache when both fit
+        {
+            const int na = SA.type == D3D_HULL ? SA.nv : 0;
+            const int nb = SB.type == D3D_HULL ? SB.nv : 0;
+            const bool fit = na + nb <= kEpaCache;
+            if (fit) {
+                for (int j = lane; j < na; j += kEpaBlock) {
+                    L.cx[j] = s.vx[SA.voff + j];
+                    L.cy[j] = s.vy[SA.voff + j];
+                    L.cz[j] = s.vz[SA.voff + j];
+                }
+                for (int j = lane; j < nb; j += kEpaBlock) {
+                    L.cx[na + j] = s.vx[SB.voff + j];
+                    L.cy[na + j] = s.vy[SB.voff + j];
+                    L.cz[na + j] = s.vz[SB.voff + j];
+                }
+            }
+            if (lane == 0) {
+                L.coff[0] = fit && na > 0 ? 0 : -1;
+                L.coff[1] = fit && nb > 0 ? na : -1;
+            }
+        }
         // ---- seed: the GJK simplex rows (w, support on A, support on B) go
         // straight into polytope rows 0..3
         const double *S = simplex + 36 * pid;
@@ -267,7 +358,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             kb = bk;
             const v3 cn = mk(L.fnx[kb], L.fny[kb], L.fnz[kb]);
             const double cdist = L.fd[kb];
-            const SupportPair sp = support_pair<kWave, true>(s, ia, ib, cn, lane);
+            const SupportPair sp = epa_support_pair(s, L, ia, ib, cn);
             const v3 spa = sp.a, spb = sp.b;
             const v3 w = spa - spb;
             const double wn = dot(w, cn);

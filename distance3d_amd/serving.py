@@ -275,8 +275,8 @@ class CollisionServer:
             while i < n and reqs[i].arrival <= now:
                 pending.append(reqs[i])
                 i += 1
-            while busy and busy[0].done():
-                s = busy.popleft()
+            for s in [b for b in busy if b.done()]:
+                busy.remove(s)
                 start = first_row[s.batch[0][0].rid]
                 for res in s.results(clock() - t0 if realtime else 0.0, arena, start):
                     results[res.rid] = res
