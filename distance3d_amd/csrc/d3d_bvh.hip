@@ -101,6 +101,18 @@ inline BvhView bvh_view(void *base, int64_t n) {
     return v;
 }
 
+static int cu_count_bvh() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
 // ------------------------------------------------------------- bounds
 
 __global__ __launch_bounds__(kBvhBlock) void bvh_bounds_partial(const double *__restrict__ aabbs,
@@ -306,14 +318,25 @@ __device__ __forceinline__ void store_box(BvhNode *nd, const float lo[3], const 
     nd->hi[2] = hi[2];
 }
 
-// leaves get their outward-rounded FP32 box, then climb: the second child
-// to arrive at a node computes its box (union of the children).
-__global__ __launch_bounds__(kBvhBlock) void bvh_refit_kernel(const double *__restrict__ aabbs,
+// Refit, level by level without device-wide fences: the leaf launch writes
+// the leaf boxes; each internal node keeps an arrival counter (flags) and the
+// child that arrives second appends the node to the next frontier list.  A
+// level launch gives every node of its frontier the union of its children's
+// boxes (written by earlier launches, so visible) and builds the next
+// frontier the same way.  A node splits its leaf range at the highest
+// differing bit of (30-bit Morton code, leaf index), so the tree has at most
+// 60 internal levels for n < 2^30 and kRefitLevels launches reach the root.
+constexpr int kRefitLevels = 62;
+
+__global__ __launch_bounds__(kBvhBlock) void bvh_refit_leaves(const double *__restrict__ aabbs,
                                                               int64_t n,
                                                               const int32_t *__restrict__ sorted_vals,
                                                               int32_t *__restrict__ leaf_orig,
-                                                              BvhNode *nodes, const int32_t *parent,
-                                                              const int32_t *right, int32_t *flags) {
+                                                              BvhNode *__restrict__ nodes,
+                                                              const int32_t *__restrict__ parent,
+                                                              int32_t *__restrict__ flags,
+                                                              int32_t *__restrict__ front,
+                                                              int32_t *__restrict__ count) {
     const int64_t i = (int64_t)blockIdx.x * kBvhBlock + threadIdx.x;
     if (i >= n) return;
     const int orig = sorted_vals[i];
@@ -328,24 +351,36 @@ __global__ __launch_bounds__(kBvhBlock) void bvh_refit_kernel(const double *__re
     store_box(&nodes[leaf], lo, hi);
     nodes[leaf].left = -1;
     if (n == 1) return;
-    __threadfence();
-    int p = parent[leaf];
-    // at most n-1 internal ancestors; the cap only guards a malformed tree
-    for (int64_t steps = 0; p >= 0 && steps < n; ++steps) {
-        if (atomicAdd(&flags[p], 1) == 0) return;
-        __threadfence();
-        const int a = nodes[p].left, b = right[p];
-        const volatile BvhNode *na = &nodes[a];
-        const volatile BvhNode *nb = &nodes[b];
+    const int p = parent[leaf];
+    if (p >= 0 && atomicAdd(&flags[p], 1) == 1) front[atomicAdd(count, 1)] = p;
+}
+
+__global__ __launch_bounds__(kBvhBlock) void bvh_refit_level(BvhNode *__restrict__ nodes,
+                                                             const int32_t *__restrict__ parent,
+                                                             const int32_t *__restrict__ right,
+                                                             int32_t *__restrict__ flags,
+                                                             const int32_t *__restrict__ front_in,
+                                                             const int32_t *__restrict__ count_in,
+                                                             int32_t *__restrict__ front_out,
+                                                             int32_t *__restrict__ count_out) {
+    const int cnt = *count_in;
+    for (int i = blockIdx.x * kBvhBlock + threadIdx.x; i < cnt; i += gridDim.x * kBvhBlock) {
+        const int p = front_in[i];
+        const BvhNode na = nodes[nodes[p].left], nb = nodes[right[p]];
+        float lo[3], hi[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(na->lo[k], nb->lo[k]);
-            hi[k] = fmaxf(na->hi[k], nb->hi[k]);
+            lo[k] = fminf(na.lo[k], nb.lo[k]);
+            hi[k] = fmaxf(na.hi[k], nb.hi[k]);
         }
         store_box(&nodes[p], lo, hi);
-        __threadfence();
-        p = parent[p];
+        const int q = parent[p];
+        if (q >= 0 && atomicAdd(&flags[q], 1) == 1) front_out[atomicAdd(count_out, 1)] = q;
     }
+}
+
+__global__ void bvh_zero_ints(int32_t *__restrict__ a, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
 }
 
 __global__ __launch_bounds__(kBvhBlock) void bvh_rope_kernel(int64_t n, BvhNode *__restrict__ nodes,
@@ -462,17 +497,33 @@ extern "C" int d3d_bvh_build(const double *aabbs, int64_t n, void *bvh, int64_t 
         uint32_t *tk = kin; kin = kout; kout = tk;
         int32_t *tv = vin; vin = vout; vout = tv;
     }
-    // after 4 passes the sorted data is back in keys / leaf_orig's buffer;
-    // the refit copies the permutation into leaf_orig (same buffer: no-op copy)
+    // after 4 passes the sorted data is back in keys / leaf_orig's buffer
     if (n > 1) {
         const int ni = (int)((n - 1 + kBvhBlock - 1) / kBvhBlock);
         hipLaunchKernelGGL(bvh_hierarchy_kernel, dim3(ni), dim3(kBvhBlock), 0, st, kin, n, v.nodes,
                            v.parent, v.right, v.flags);
         D3D_CHECK_LAUNCH("bvh_hierarchy_kernel");
     }
-    hipLaunchKernelGGL(bvh_refit_kernel, dim3(nb), dim3(kBvhBlock), 0, st, aabbs, n, vin, v.leaf_orig,
-                       v.nodes, v.parent, v.right, v.flags);
-    D3D_CHECK_LAUNCH("bvh_refit_kernel");
+    // frontiers ping-pong between keys_alt and vals_alt (free after the sort);
+    // per-level counters live in the histogram area
+    int32_t *fr[2] = {(int32_t *)v.keys_alt, v.vals_alt};
+    int32_t *cnt = v.hist;
+    hipLaunchKernelGGL(bvh_zero_ints, dim3(1), dim3(kBvhBlock), 0, st, cnt, kRefitLevels + 1);
+    D3D_CHECK_LAUNCH("bvh_zero_ints");
+    hipLaunchKernelGGL(bvh_refit_leaves, dim3(nb), dim3(kBvhBlock), 0, st, aabbs, n, vin, v.leaf_orig,
+                       v.nodes, v.parent, v.flags, fr[0], cnt);
+    D3D_CHECK_LAUNCH("bvh_refit_leaves");
+    if (n > 1) {
+        int lg = (int)((n / 2 + kBvhBlock - 1) / kBvhBlock);
+        const int cap = 4 * cu_count_bvh();
+        if (lg > cap) lg = cap;
+        if (lg < 1) lg = 1;
+        for (int L = 0; L < kRefitLevels; ++L) {
+            hipLaunchKernelGGL(bvh_refit_level, dim3(lg), dim3(kBvhBlock), 0, st, v.nodes, v.parent,
+                               v.right, v.flags, fr[L & 1], cnt + L, fr[(L + 1) & 1], cnt + L + 1);
+            D3D_CHECK_LAUNCH("bvh_refit_level");
+        }
+    }
     // ropes; the refit flags (no longer needed) hold the left-child array
     if (n > 1) {
         hipLaunchKernelGGL(bvh_left_kernel, dim3((unsigned)((n - 1 + kBvhBlock - 1) / kBvhBlock)),
