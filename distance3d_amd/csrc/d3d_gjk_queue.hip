@@ -409,31 +409,40 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
     int ia = 0, ib = 0;
     int pid = 0;
     bool have = false, exhausted = false;
-    constexpr int kChunk = 2 * (kWave / G);
+    // Items per claim: 2 * (64 / G), or the wavefront's fair share of the
+    // class when that is smaller, so that a small batch spreads its pairs over
+    // all wavefronts (one slow pair per wavefront instead of a few wavefronts
+    // holding every pair of the class).
+    const int n_waves = (int)gridDim.x * (kQBlock / kWave);
+    const int fair = (count + n_waves - 1) / n_waves;
+    const int chunk = fair < 2 * (kWave / G) ? (fair < 1 ? 1 : fair) : 2 * (kWave / G);
     int cb = 0, ce = 0;  // the wavefront's unclaimed items [cb, ce) of the class
     while (true) {
         const bool need = !have && !exhausted;
         const unsigned long long m = __ballot(need && sub == 0);
         if (m != 0ull) {
             // groups take items from the wavefront's chunk of the class range;
-            // one atomic fetches the next chunk of kChunk items when it runs out
+            // one atomic fetches the next chunk when it runs out.  Groups
+            // ranked past the items at hand wait for the next round.
             const int c = __popcll(m);
             const int rank = __shfl(__popcll(m & lanemask_lt(lane)), leader);
             const int avail = ce - cb;
             int nb = 0;
             if (avail < c) {
                 const int first = __ffsll((long long)m) - 1;
-                if (lane == first) nb = atomicAdd(cursor, kChunk);
+                if (lane == first) nb = atomicAdd(cursor, chunk);
                 nb = readlane_i32(nb, first);
             }
             const int base = rank < avail ? cb : nb - avail;
+            const bool served = rank < avail + (avail < c ? chunk : 0);
             if (avail < c) {
-                cb = nb + (c - avail);
-                ce = nb + kChunk;
+                const int taken = c - avail < chunk ? c - avail : chunk;
+                cb = nb + taken;
+                ce = nb + chunk;
             } else {
                 cb += c;
             }
-            if (need) {
+            if (need && served) {
                 const int idx = base + rank;
                 if (idx < count) {
                     pid = order[begin + idx];
@@ -454,6 +463,9 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                     exhausted = true;
                 }
             }
+            // the cursor only grows: once any group sees an item past the end
+            // the class is empty for every group of the wavefront
+            if (__ballot(exhausted) != 0ull) exhausted = true;
         }
         if (__ballot(have) == 0ull) {
             if (__ballot(!exhausted) == 0ull) break;

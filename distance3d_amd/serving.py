@@ -21,6 +21,7 @@ Scheduling:
   pair is computed independently, in FP64, by the same kernels.
 """
 import collections
+import gc
 import time
 
 import numpy as np
@@ -261,6 +262,18 @@ class CollisionServer:
         for r in reqs:
             first_row[r.rid] = row
             row += r.n_pairs
+        # a full garbage collection while batches are in flight stalls every
+        # slot at once (tens of ms with the stream's result objects alive), so
+        # the collector is paused for the duration of the stream
+        gc_was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            return self._serve(reqs, arena, first_row, realtime, clock)
+        finally:
+            if gc_was_enabled:
+                gc.enable()
+
+    def _serve(self, reqs, arena, first_row, realtime, clock):
         pending = collections.deque()
         free = collections.deque(self.slots)
         busy = collections.deque()
