@@ -256,13 +256,25 @@ __device__ __forceinline__ void top2(float sc, int j, float &m1, float &m2, int 
 // below the exact score of the winner (>= m1 - E), so the certificate keeps
 // its meaning.  The result equals the full sweep's bit for bit.
 // FP64 argmax over the hull by the G lanes of a group, ties to the smallest
-// original index (vorig): the exact fallback of the FP32 certificate.
+// original index (vorig): the exact fallback of the FP32 certificate.  Only
+// vertices whose FP32 score reaches thr = rd(m1 - 2.5E) are evaluated in FP64
+// (clusters whose bound is below thr are skipped): the FP64 argmax j* and
+// every vertex tied with it score at least m1 - E - 2d exactly (d: FP64
+// rounding, far below E), hence at least m1 - 2E - 2d in FP32, so no
+// candidate is missed.  A NaN or -inf threshold evaluates every vertex.
 template <int G>
-__device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H, v3 l, int sub) {
+__device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H, v3 l, int sub,
+                                           float thr) {
+    constexpr int K = D3D_CLUSTER_SIZE;
+    const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
+    const float *__restrict__ px = s.vx + H.voff;
+    const float *__restrict__ py = s.vy + H.voff;
+    const float *__restrict__ pz = s.vz + H.voff;
     const int nv = H.nv;
+    if (thr != thr) thr = -INFINITY;
     double best = -INFINITY;
     int bi = 0x7fffffff, bo = 0x7fffffff;
-    for (int j = sub; j < nv; j += G) {
+    auto consider = [&](int j) {
         const v3 v = hull_vertex(s, H.voff, H.voff64, j);
         const double val = v.x * l.x + v.y * l.y + v.z * l.z;
         const int oj = s.vorig ? s.vorig[H.voff + j] : j;
@@ -270,6 +282,25 @@ __device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H,
             best = val;
             bi = j;
             bo = oj;
+        }
+    };
+    auto score = [&](int j) {
+        return __builtin_fmaf(pz[j], dz, __builtin_fmaf(py[j], dy, px[j] * dx));
+    };
+    if (H.nc == 0) {
+        for (int j = sub; j < nv; j += G)
+            if (!(score(j) < thr)) consider(j);
+    } else {
+        const float4 *__restrict__ cl = reinterpret_cast<const float4 *>(s.clusters) + H.coff;
+        const float nl = (float)(sqrt(l.x * l.x + l.y * l.y + l.z * l.z) * (1.0 + 0x1p-19));
+        for (int k = sub; k < H.nc; k += G) {
+            const float4 c = cl[k];
+            if (__builtin_fmaf(c.w, nl, __builtin_fmaf(c.z, dz, __builtin_fmaf(c.y, dy, c.x * dx))) < thr)
+                continue;
+            const int j0 = k * K;
+            const int je = min(j0 + K, nv);
+            for (int j = j0; j < je; ++j)
+                if (!(score(j) < thr)) consider(j);
         }
     }
 #pragma unroll
@@ -385,7 +416,7 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
     }
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
 
-    return hull_rescan<G>(s, H, l, sub);
+    return hull_rescan<G>(s, H, l, sub, __double2float_rd((double)m1 - 2.5 * E));
 }
 
 // Closed-form supports, FP64, in the same operation order as the reference.

@@ -68,7 +68,7 @@ __device__ __forceinline__ int lds_hull_argmax(const d3d_scene &s, const EpaLds 
         i1 = take ? oi : i1;
     }
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
-    return hull_rescan<kWave>(s, H, l, lane);
+    return hull_rescan<kWave>(s, H, l, lane, __double2float_rd((double)m1 - 2.5 * E));
 }
 
 // Supports of record ia along d and of record ib along -d for the EPA

@@ -398,7 +398,7 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                                           const int32_t *__restrict__ pb,
                                           const int32_t *__restrict__ order, int begin, int count,
                                           int32_t *__restrict__ cursor, const d3d_gjk_params &prm,
-                                          const GjkOut &o, double *pq_wave) {
+                                          const GjkOut &o, double *pq_wave, int chunk, int rot) {
     const int lane = lane_id();
     const int sub = lane & (G - 1);
     const int leader = lane - sub;
@@ -409,14 +409,19 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
     int ia = 0, ib = 0;
     int pid = 0;
     bool have = false, exhausted = false;
-    // Items per claim: 2 * (64 / G), or the wavefront's fair share of the
-    // class when that is smaller, so that a small batch spreads its pairs over
-    // all wavefronts (one slow pair per wavefront instead of a few wavefronts
-    // holding every pair of the class).
+    // The first chunk of every wavefront is fixed by its rank (wavefront index
+    // minus `rot`, modulo the wavefront count): no atomic, so thousands of
+    // wavefronts do not queue on one cursor when a launch starts, and the
+    // rotation hands each class's fixed chunks to the wavefronts the earlier
+    // classes left without one.  Later chunks come from the cursor, past all
+    // fixed chunks; when the fixed chunks cover the class there are none.
     const int n_waves = (int)gridDim.x * (kQBlock / kWave);
-    const int fair = (count + n_waves - 1) / n_waves;
-    const int chunk = fair < 2 * (kWave / G) ? (fair < 1 ? 1 : fair) : 2 * (kWave / G);
-    int cb = 0, ce = 0;  // the wavefront's unclaimed items [cb, ce) of the class
+    const int wv = (int)blockIdx.x * (kQBlock / kWave) + (int)threadIdx.x / kWave;
+    const int64_t r = (wv - rot + n_waves) % n_waves;
+    const int64_t fixed_end = (int64_t)n_waves * chunk;
+    const bool dynamic = fixed_end < count;
+    int cb = (int)min(r * chunk, (int64_t)count), ce = cb + chunk;
+    // the wavefront's unclaimed items are [cb, ce) of the class
     while (true) {
         const bool need = !have && !exhausted;
         const unsigned long long m = __ballot(need && sub == 0);
@@ -430,8 +435,12 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
             int nb = 0;
             if (avail < c) {
                 const int first = __ffsll((long long)m) - 1;
-                if (lane == first) nb = atomicAdd(cursor, chunk);
-                nb = readlane_i32(nb, first);
+                if (dynamic) {
+                    if (lane == first) nb = (int)fixed_end + atomicAdd(cursor, chunk);
+                    nb = readlane_i32(nb, first);
+                } else {
+                    nb = count;  // nothing past the fixed chunks
+                }
             }
             const int base = rank < avail ? cb : nb - avail;
             const bool served = rank < avail + (avail < c ? chunk : 0);
@@ -502,27 +511,37 @@ __global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
     GjkOut o) {
     __shared__ double pq_lds[24 * kQBlock];
     double *pq_wave = pq_lds + 24 * kWave * (threadIdx.x / kWave);
+    const int n_waves = (int)gridDim.x * (kQBlock / kWave);
+    int rot = 0;  // wavefronts given a fixed chunk by the classes walked so far
     for (int cls = kNumClasses - 1; cls >= 0; --cls) {
         const int count = cls_count[cls];
         if (count == 0) continue;
         const int begin = cls_begin[cls];
         int32_t *cur = cls_cursor + cls;
         const int G = prm.lanes_per_pair[cls];
+        // items per claim: 2 * (64 / G), or the class's fair share per
+        // wavefront when that is smaller, so a small batch spreads its pairs
+        // over all wavefronts instead of a few wavefronts holding all of them
+        const int full = 2 * (kWave / G);
+        const int fair = (count + n_waves - 1) / n_waves;
+        const int chunk = fair < full ? fair : full;
         if (cls == 0) {
-            if (G <= 1) run_class<1, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave);
-            else if (G == 2) run_class<2, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave);
-            else run_class<4, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave);
+            if (G <= 1) run_class<1, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot);
+            else if (G == 2) run_class<2, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot);
+            else run_class<4, false>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot);
         } else {
             switch (G) {
-            case 1: run_class<1, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
-            case 2: run_class<2, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
-            case 4: run_class<4, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
-            case 8: run_class<8, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
-            case 16: run_class<16, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
-            case 32: run_class<32, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
-            default: run_class<64, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave); break;
+            case 1: run_class<1, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot); break;
+            case 2: run_class<2, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot); break;
+            case 4: run_class<4, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot); break;
+            case 8: run_class<8, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot); break;
+            case 16: run_class<16, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot); break;
+            case 32: run_class<32, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot); break;
+            default: run_class<64, true>(s, pa, pb, order, begin, count, cur, prm, o, pq_wave, chunk, rot); break;
             }
         }
+        const int used = (count + chunk - 1) / chunk;
+        rot = (int)(((int64_t)rot + (used < n_waves ? used : n_waves)) % n_waves);
     }
 }
 
@@ -633,8 +652,7 @@ static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, 
         if (c == 0 && G > 4) G = 4;
         rp.lanes_per_pair[c] = G;
     }
-    // resident at occupancy 3 (4 wavefronts per block); wavefronts without work
-    // leave after one claim per class
+    // resident at occupancy 3 (4 wavefronts per block)
     const int64_t grid = (int64_t)cu_count() * 3;
     hipLaunchKernelGGL(gjk_all_kernel, dim3((unsigned)grid), dim3(kQBlock), 0, stream, s, pa, pb,
                        w.order, w.cls_begin, w.cls_count, w.cls_cursor, rp, o);
