@@ -305,16 +305,21 @@ def run_broad(args):
 
 
 def run_serving(args, lanes):
-    from distance3d_amd import serving, workloads
+    from distance3d_amd import autotune, serving, workloads
     t0 = time.time()
     scene, poses, reqs = workloads.serving_workload(n_requests=args.serving_requests)
     log("serving workload in %.1fs: %d requests, %d pairs"
         % (time.time() - t0, len(reqs), sum(r.n_pairs for r in reqs)))
     dev = scene.to_device("cuda:0")
-    srv = serving.CollisionServer(dev, poses, n_slots=3, slot_pairs=16384,
-                                  lanes_per_pair=lanes)
+    srv = serving.CollisionServer(dev, poses, n_slots=4, slot_pairs=16384,
+                                  lanes_per_pair=autotune.tuned_lanes(kind="latency"))
     log("server ready (graphs captured)")
+    # warm-up: the first 2000 requests of the stream, served and discarded
+    srv.serve(reqs[:2000], realtime=True)
     res, stats = srv.serve(reqs, realtime=True)
+    stats["warmup_requests"] = 2000
+    stats["slots"] = 4
+    stats["lanes_per_pair"] = srv.lanes_per_pair
     stats["workload"] = ("Poisson stream, %d requests at 20k req/s, log-uniform 1..4096 "
                          "pose-updated pairs, 512-collider resident scene" % len(reqs))
     stats["slot_bytes"] = serving.CollisionServer.slot_bytes(16384)

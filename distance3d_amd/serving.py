@@ -215,11 +215,13 @@ class GpuSlot:
 class CollisionServer:
     """Continuous batching over a device-resident scene and pose table."""
 
-    def __init__(self, scene, poses, n_slots=3, slot_pairs=16384, lanes_per_pair=None,
+    def __init__(self, scene, poses, n_slots=4, slot_pairs=16384, lanes_per_pair=None,
                  use_graphs=True, slot_factory=None):
         """``scene``: DeviceScene; ``poses``: (P, 12) float64 pose rows (R row
         major, t), kept on the device with two extra rows used to pad
-        batches (a pair placed 1e4 apart, clipped in one GJK iteration)."""
+        batches (a pair placed 1e4 apart, clipped in one GJK iteration).
+        ``lanes_per_pair``: per work class; None takes the autotuner's
+        latency table."""
         import torch
         self.scene = scene
         P = np.asarray(poses, dtype=np.float64).reshape(-1, 12)
@@ -232,6 +234,11 @@ class CollisionServer:
         cost = np.where(i32[:, 0] == 0, i32[:, 1], 0)
         k = int(np.argmin(cost))
         self.dummy = (k, k, len(P), len(P) + 1)
+        if lanes_per_pair is None:
+            # the autotuner's latency table (lanes per pair that minimise the
+            # time of one small batch), else the library defaults
+            from . import autotune
+            lanes_per_pair = autotune.tuned_lanes(kind="latency")
         self.lanes_per_pair = lanes_per_pair
         factory = slot_factory or (lambda srv, cap: GpuSlot(srv, cap, use_graphs))
         self.slots = [factory(self, slot_pairs) for _ in range(n_slots)]
