@@ -255,6 +255,46 @@ __device__ __forceinline__ void top2(float sc, int j, float &m1, float &m2, int 
 // rounds per sweep): a skipped vertex has exact score below m1 - 2.5E, hence
 // below the exact score of the winner (>= m1 - E), so the certificate keeps
 // its meaning.  The result equals the full sweep's bit for bit.
+// Running FP64 argmax with ties to the smallest original index.
+struct ArgmaxF64 {
+    double best;
+    int bi, bo;
+    __device__ __forceinline__ void init() {
+        best = -INFINITY;
+        bi = 0x7fffffff;
+        bo = 0x7fffffff;
+    }
+    __device__ __forceinline__ void take(double val, int j, int oj) {
+        if (val > best || (val == best && oj < bo)) {
+            best = val;
+            bi = j;
+            bo = oj;
+        }
+    }
+    // merge over the G lanes of a group; the index (0 when nothing was taken)
+    template <int G>
+    __device__ __forceinline__ int group_result() {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            const int oo = __shfl_xor(bo, off);
+            const bool t = (ob > best) || (ob == best && oo < bo);
+            best = t ? ob : best;
+            bi = t ? oi : bi;
+            bo = t ? oo : bo;
+        }
+        return bi == 0x7fffffff ? 0 : bi;
+    }
+};
+
+// FP64 score of vertex j with its original index, into the running argmax.
+__device__ __forceinline__ void rescan_take(const d3d_scene &s, const HullRef &H, v3 l, int j,
+                                            ArgmaxF64 &A) {
+    const v3 v = hull_vertex(s, H.voff, H.voff64, j);
+    A.take(v.x * l.x + v.y * l.y + v.z * l.z, j, s.vorig ? s.vorig[H.voff + j] : j);
+}
+
 // FP64 argmax over the hull by the G lanes of a group, ties to the smallest
 // original index (vorig): the exact fallback of the FP32 certificate.  Only
 // vertices whose FP32 score reaches thr = rd(m1 - 2.5E) are evaluated in FP64
@@ -272,24 +312,14 @@ __device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H,
     const float *__restrict__ pz = s.vz + H.voff;
     const int nv = H.nv;
     if (thr != thr) thr = -INFINITY;
-    double best = -INFINITY;
-    int bi = 0x7fffffff, bo = 0x7fffffff;
-    auto consider = [&](int j) {
-        const v3 v = hull_vertex(s, H.voff, H.voff64, j);
-        const double val = v.x * l.x + v.y * l.y + v.z * l.z;
-        const int oj = s.vorig ? s.vorig[H.voff + j] : j;
-        if (val > best || (val == best && oj < bo)) {
-            best = val;
-            bi = j;
-            bo = oj;
-        }
-    };
+    ArgmaxF64 A;
+    A.init();
     auto score = [&](int j) {
         return __builtin_fmaf(pz[j], dz, __builtin_fmaf(py[j], dy, px[j] * dx));
     };
     if (H.nc == 0) {
         for (int j = sub; j < nv; j += G)
-            if (!(score(j) < thr)) consider(j);
+            if (!(score(j) < thr)) rescan_take(s, H, l, j, A);
     } else {
         const float4 *__restrict__ cl = reinterpret_cast<const float4 *>(s.clusters) + H.coff;
         const float nl = (float)(sqrt(l.x * l.x + l.y * l.y + l.z * l.z) * (1.0 + 0x1p-19));
@@ -298,22 +328,27 @@ __device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H,
             if (__builtin_fmaf(c.w, nl, __builtin_fmaf(c.z, dz, __builtin_fmaf(c.y, dy, c.x * dx))) < thr)
                 continue;
             const int j0 = k * K;
-            const int je = min(j0 + K, nv);
-            for (int j = j0; j < je; ++j)
-                if (!(score(j) < thr)) consider(j);
+            if (j0 + K <= nv) {
+                // aligned cluster: four vertices per load, loads issued together
+                const float4 *__restrict__ qx = reinterpret_cast<const float4 *>(px + j0);
+                const float4 *__restrict__ qy = reinterpret_cast<const float4 *>(py + j0);
+                const float4 *__restrict__ qz = reinterpret_cast<const float4 *>(pz + j0);
+#pragma unroll 2
+                for (int q = 0; q < K / 4; ++q) {
+                    const float4 x = qx[q], y = qy[q], z = qz[q];
+                    const int j = j0 + 4 * q;
+                    if (!(__builtin_fmaf(z.x, dz, __builtin_fmaf(y.x, dy, x.x * dx)) < thr)) rescan_take(s, H, l, j, A);
+                    if (!(__builtin_fmaf(z.y, dz, __builtin_fmaf(y.y, dy, x.y * dx)) < thr)) rescan_take(s, H, l, j + 1, A);
+                    if (!(__builtin_fmaf(z.z, dz, __builtin_fmaf(y.z, dy, x.z * dx)) < thr)) rescan_take(s, H, l, j + 2, A);
+                    if (!(__builtin_fmaf(z.w, dz, __builtin_fmaf(y.w, dy, x.w * dx)) < thr)) rescan_take(s, H, l, j + 3, A);
+                }
+            } else {
+                for (int j = j0; j < nv; ++j)
+                    if (!(score(j) < thr)) rescan_take(s, H, l, j, A);
+            }
         }
     }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        const int oo = __shfl_xor(bo, off);
-        const bool take = (ob > best) || (ob == best && oo < bo);
-        best = take ? ob : best;
-        bi = take ? oi : bi;
-        bo = take ? oo : bo;
-    }
-    return bi == 0x7fffffff ? 0 : bi;
+    return A.group_result<G>();
 }
 
 template <int G>

@@ -68,7 +68,17 @@ __device__ __forceinline__ int lds_hull_argmax(const d3d_scene &s, const EpaLds 
         i1 = take ? oi : i1;
     }
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
-    return hull_rescan<kWave>(s, H, l, lane, __double2float_rd((double)m1 - 2.5 * E));
+    // the FP64 rescan of hull_rescan, with the candidates found in the cache
+    float thr = __double2float_rd((double)m1 - 2.5 * E);
+    if (thr != thr) thr = -INFINITY;
+    ArgmaxF64 A;
+    A.init();
+    for (int j = lane; j < H.nv; j += kWave) {
+        const float sc =
+            __builtin_fmaf(L.cz[off + j], dz, __builtin_fmaf(L.cy[off + j], dy, L.cx[off + j] * dx));
+        if (!(sc < thr)) rescan_take(s, H, l, j, A);
+    }
+    return A.group_result<kWave>();
 }
 
 // Supports of record ia along d and of record ib along -d for the EPA
