@@ -20,6 +20,18 @@ import os
 import sys
 import time
 
+# HIP places the process's streams round-robin on GPU_MAX_HW_QUEUES hardware
+# queues (4 by default).  The serving leg keeps four batch slots in flight on
+# four streams beside the default stream; with fewer queues than streams two
+# slots share a queue and one batch waits behind the other.  Raised (never
+# lowered) before anything initialises HIP.
+try:
+    _queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+except ValueError:
+    _queues = 4
+if _queues < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))

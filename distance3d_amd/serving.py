@@ -19,10 +19,17 @@ Scheduling:
   costs one graph replay.
 * Results of a pair do not depend on the other pairs of its batch: every
   pair is computed independently, in FP64, by the same kernels.
+* Each slot's stream needs a hardware queue of its own: HIP places streams
+  round-robin on ``GPU_MAX_HW_QUEUES`` queues (4 by default), and two slots on
+  one queue run their batches one after the other.  Set it to at least
+  ``n_slots + 2`` in the environment before the process first uses the GPU
+  (the server warns when it finds a smaller value).
 """
 import collections
 import gc
+import os
 import time
+import warnings
 
 import numpy as np
 
@@ -240,6 +247,11 @@ class CollisionServer:
             from . import autotune
             lanes_per_pair = autotune.tuned_lanes(kind="latency")
         self.lanes_per_pair = lanes_per_pair
+        queues = os.environ.get("GPU_MAX_HW_QUEUES")
+        if slot_factory is None and (queues is None or int(queues) < n_slots + 2):
+            warnings.warn("GPU_MAX_HW_QUEUES=%s: fewer hardware queues than the %d slot streams "
+                          "need (n_slots + 2); slots will share queues and serialize"
+                          % (queues or "4 (default)", n_slots))
         factory = slot_factory or (lambda srv, cap: GpuSlot(srv, cap, use_graphs))
         self.slots = [factory(self, slot_pairs) for _ in range(n_slots)]
         for s in self.slots:
