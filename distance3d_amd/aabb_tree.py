@@ -69,6 +69,17 @@ class AabbTree:
             self._bvh = kernels.bvh_build(self._leaf_aabbs)
         return self._bvh
 
+    def update_leaf_aabbs(self, aabbs):
+        """New boxes for the leaves, in insertion order (same count): the
+        device tree, when built, is refitted instead of rebuilt."""
+        aabbs = np.asarray(aabbs, dtype=np.float64).reshape(-1, 3, 2)
+        if len(aabbs) != len(self._leaf_rows):
+            raise ValueError("expected %d boxes, got %d" % (len(self._leaf_rows), len(aabbs)))
+        self._leaf_aabbs = aabbs.copy()
+        self.aabbs[self._leaf_rows] = aabbs
+        if self._bvh is not None:
+            kernels.bvh_refit(self._bvh, self._leaf_aabbs)
+
     def overlaps_aabb(self, aabb):
         """(is_overlapping, leaf rows overlapping ``aabb``)."""
         if len(self._leaf_rows) == 0:

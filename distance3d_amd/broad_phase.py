@@ -61,14 +61,24 @@ class BoundingVolumeHierarchy:
         self.aabbtree_.insert_aabb(collider.aabb(), (frame, collider))
 
     def update_collider_poses(self):
-        """Update poses of all colliders from the transform manager and
-        rebuild the tree (broad_phase.py:144-151)."""
-        self.aabbtree_ = AabbTree()
-        for frame in self.colliders_:
+        """Update poses of all colliders from the transform manager
+        (broad_phase.py:144-151).  The reference rebuilds the tree; when the
+        tree already holds exactly these colliders in this order, its leaves
+        get the new boxes and the device tree is refitted instead."""
+        frames = list(self.colliders_)
+        aabbs = []
+        for frame in frames:
             A2B = self.tm.get_transform(frame, "origin")
             collider = self.colliders_[frame]
             collider.update_pose(A2B)
-            self.aabbtree_.insert_aabb(collider.aabb(), (frame, collider))
+            aabbs.append(collider.aabb())
+        held = [d[0] for d in self.aabbtree_.external_data_list if d is not None]
+        if held == frames and len(frames) > 0:
+            self.aabbtree_.update_leaf_aabbs(aabbs)
+            return
+        self.aabbtree_ = AabbTree()
+        for frame, aabb in zip(frames, aabbs):
+            self.aabbtree_.insert_aabb(aabb, (frame, self.colliders_[frame]))
 
     def get_colliders(self):
         return self.colliders_.values()

@@ -379,8 +379,35 @@ __global__ __launch_bounds__(kBvhBlock) void bvh_refit_level(BvhNode *__restrict
     }
 }
 
-__global__ void bvh_zero_ints(int32_t *__restrict__ a, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
+__global__ void bvh_zero_ints(int32_t *__restrict__ a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+// Boxes of every node from the leaf AABBs: leaf launch, then one launch per
+// tree level (see bvh_refit_leaves).  `flags` must be zero for the internal
+// nodes; leaf_orig holds the sorted permutation (read and rewritten in place
+// when sorted_vals == leaf_orig).
+static void refit_launches(const BvhView &v, const double *aabbs, int64_t n,
+                           const int32_t *sorted_vals, hipStream_t st) {
+    // frontiers ping-pong between keys_alt and vals_alt (free after the sort);
+    // per-level counters live in the histogram area
+    int32_t *fr[2] = {(int32_t *)v.keys_alt, v.vals_alt};
+    int32_t *cnt = v.hist;
+    const int nb = (int)((n + kBvhBlock - 1) / kBvhBlock);
+    hipLaunchKernelGGL(bvh_zero_ints, dim3(1), dim3(kBvhBlock), 0, st, cnt, (int64_t)(kRefitLevels + 1));
+    hipLaunchKernelGGL(bvh_refit_leaves, dim3(nb), dim3(kBvhBlock), 0, st, aabbs, n, sorted_vals,
+                       v.leaf_orig, v.nodes, v.parent, v.flags, fr[0], cnt);
+    if (n > 1) {
+        int lg = (int)((n / 2 + kBvhBlock - 1) / kBvhBlock);
+        const int cap = 4 * cu_count_bvh();
+        if (lg > cap) lg = cap;
+        if (lg < 1) lg = 1;
+        for (int L = 0; L < kRefitLevels; ++L)
+            hipLaunchKernelGGL(bvh_refit_level, dim3(lg), dim3(kBvhBlock), 0, st, v.nodes, v.parent,
+                               v.right, v.flags, fr[L & 1], cnt + L, fr[(L + 1) & 1], cnt + L + 1);
+    }
 }
 
 __global__ __launch_bounds__(kBvhBlock) void bvh_rope_kernel(int64_t n, BvhNode *__restrict__ nodes,
@@ -504,26 +531,8 @@ extern "C" int d3d_bvh_build(const double *aabbs, int64_t n, void *bvh, int64_t 
                            v.parent, v.right, v.flags);
         D3D_CHECK_LAUNCH("bvh_hierarchy_kernel");
     }
-    // frontiers ping-pong between keys_alt and vals_alt (free after the sort);
-    // per-level counters live in the histogram area
-    int32_t *fr[2] = {(int32_t *)v.keys_alt, v.vals_alt};
-    int32_t *cnt = v.hist;
-    hipLaunchKernelGGL(bvh_zero_ints, dim3(1), dim3(kBvhBlock), 0, st, cnt, kRefitLevels + 1);
-    D3D_CHECK_LAUNCH("bvh_zero_ints");
-    hipLaunchKernelGGL(bvh_refit_leaves, dim3(nb), dim3(kBvhBlock), 0, st, aabbs, n, vin, v.leaf_orig,
-                       v.nodes, v.parent, v.flags, fr[0], cnt);
-    D3D_CHECK_LAUNCH("bvh_refit_leaves");
-    if (n > 1) {
-        int lg = (int)((n / 2 + kBvhBlock - 1) / kBvhBlock);
-        const int cap = 4 * cu_count_bvh();
-        if (lg > cap) lg = cap;
-        if (lg < 1) lg = 1;
-        for (int L = 0; L < kRefitLevels; ++L) {
-            hipLaunchKernelGGL(bvh_refit_level, dim3(lg), dim3(kBvhBlock), 0, st, v.nodes, v.parent,
-                               v.right, v.flags, fr[L & 1], cnt + L, fr[(L + 1) & 1], cnt + L + 1);
-            D3D_CHECK_LAUNCH("bvh_refit_level");
-        }
-    }
+    refit_launches(v, aabbs, n, vin, st);
+    D3D_CHECK_LAUNCH("bvh_refit");
     // ropes; the refit flags (no longer needed) hold the left-child array
     if (n > 1) {
         hipLaunchKernelGGL(bvh_left_kernel, dim3((unsigned)((n - 1 + kBvhBlock - 1) / kBvhBlock)),
@@ -534,6 +543,26 @@ extern "C" int d3d_bvh_build(const double *aabbs, int64_t n, void *bvh, int64_t 
     hipLaunchKernelGGL(bvh_rope_kernel, dim3(nn), dim3(kBvhBlock), 0, st, n, v.nodes, v.parent,
                        v.flags, v.right);
     D3D_CHECK_LAUNCH("bvh_rope_kernel");
+    return D3D_OK;
+}
+
+extern "C" int d3d_bvh_refit(const double *aabbs, int64_t n, void *bvh, int64_t bvh_bytes_avail,
+                             void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(n >= 1 && n < (int64_t)0x40000000, "n must be in [1, 2^30)");
+    D3D_CHECK_ARG(aabbs && bvh, "NULL buffer");
+    D3D_CHECK_ARG(bvh_bytes_avail >= bvh_bytes(n), "bvh buffer smaller than d3d_bvh_bytes(n)");
+    hipStream_t st = (hipStream_t)stream;
+    BvhView v = bvh_view(bvh, n);
+    // the build left the ropes' left-child array in `flags`; the arrival
+    // counters of the refit need it zeroed (the ropes are already in the nodes)
+    if (n > 1) {
+        const int64_t zb = (n - 1 + kBvhBlock - 1) / kBvhBlock;
+        hipLaunchKernelGGL(bvh_zero_ints, dim3((unsigned)(zb < 1024 ? zb : 1024)), dim3(kBvhBlock), 0, st,
+                           v.flags, n - 1);
+    }
+    refit_launches(v, aabbs, n, v.leaf_orig, st);
+    D3D_CHECK_LAUNCH("bvh_refit");
     return D3D_OK;
 }
 

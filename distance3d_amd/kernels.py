@@ -196,6 +196,19 @@ def bvh_build(aabbs, stream=None):
     return Bvh(a, buf, n)
 
 
+def bvh_refit(bvh, aabbs, stream=None):
+    """New (n,3,2) float64 boxes for the leaves of ``bvh`` (box i keeps leaf
+    slot i, the topology stays); the boxes are copied into ``bvh.aabbs``."""
+    import torch
+    a = torch.as_tensor(aabbs, dtype=torch.float64).reshape(-1, 3, 2)
+    if a.shape[0] != bvh.n:
+        raise ValueError("refit needs %d boxes, got %d" % (bvh.n, a.shape[0]))
+    bvh.aabbs.copy_(a.to(bvh.aabbs.device))
+    check(lib().d3d_bvh_refit(ptr(bvh.aabbs), bvh.n, ptr(bvh.buffer), bvh.buffer.numel(),
+                              stream_ptr(stream)), "d3d_bvh_refit")
+    return bvh
+
+
 def bvh_query(bvh, queries, cap=None, counter=None, stream=None, sort=True):
     """All (query, box) index pairs with closed-interval overlap, as an
     (k, 2) int32 device tensor (lexicographically sorted when ``sort``).

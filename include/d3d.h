@@ -287,6 +287,17 @@ int d3d_bvh_build(const double *aabbs, int64_t n, void *bvh, int64_t bvh_bytes, 
 int d3d_bvh_query(const void *bvh, int64_t n, const double *aabbs, const double *queries,
                   int64_t m, int32_t *pairs, int64_t cap, uint64_t *n_pairs, void *stream);
 
+/*
+ * d3d_bvh_refit(aabbs (n,3,2) f64, n, bvh, bytes, stream): new boxes for the
+ * leaves of a BVH that d3d_bvh_build built over n boxes (box i keeps leaf
+ * slot i), keeping its topology: the node boxes are recomputed bottom-up and
+ * queries stay exact (a node box only has to contain its leaves' boxes).
+ * `aabbs` replaces the array the build was given.  Replaces the full
+ * rebuild by serial insertion on every pose update in
+ * BoundingVolumeHierarchy.update_collider_poses (broad_phase.py:144-151).
+ */
+int d3d_bvh_refit(const double *aabbs, int64_t n, void *bvh, int64_t bvh_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

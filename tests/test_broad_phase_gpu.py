@@ -199,6 +199,45 @@ def test_bvh_with_self_matches_other_bvh(gpu):  # test_broad_phase.py:32-101
     assert sorted(map(key, self_pairs)) == expect
 
 
+def test_bvh_refit_matches_brute_force(gpu):
+    """Refitting a built tree with moved, grown and shrunk boxes gives the
+    brute-force pair sets; two refits in a row, and trees of 1 and 2 boxes."""
+    rs = np.random.RandomState(11)
+    for n in (1, 2, 3000):
+        boxes = _boxes(rs, n)
+        bvh = kernels.bvh_build(boxes)
+        queries = _boxes(rs, 500, wmax=2.0)
+        for step in range(2):
+            moved = boxes + rs.uniform(-3.0, 3.0, (n, 3, 1))
+            moved[:, :, 1] += rs.uniform(0.0, 1.0, (n, 3)) * (step == 0)
+            kernels.bvh_refit(bvh, moved)
+            got = kernels.bvh_query(bvh, queries).cpu().numpy().astype(np.int64)
+            assert np.array_equal(got, _oracle_pairs(moved, queries))
+            assert np.array_equal(got, _gpu_pairs(moved, queries))
+
+
+def test_update_collider_poses_refits(gpu):
+    """Moving frames and updating poses keeps the tree object and refits it;
+    overlaps equal brute force over the new boxes."""
+    rs = np.random.RandomState(12)
+    tm, bvh = _robotish(rs, 40)
+    bvh.aabb_overlapping_with_self()  # builds the device tree
+    tree, dev = bvh.aabbtree_, bvh.aabbtree_._bvh
+    assert dev is not None
+    for f in list(bvh.colliders_):
+        T = tm.get_transform(f, "origin").copy()
+        T[:3, 3] += rs.uniform(-0.5, 0.5, 3)
+        tm.add_transform(f, "origin", T)
+    bvh.update_collider_poses()
+    assert bvh.aabbtree_ is tree and bvh.aabbtree_._bvh is dev
+    frames = list(bvh.colliders_)
+    boxes = np.array([bvh.colliders_[f].aabb() for f in frames])
+    o, _ = oracle.all_aabbs_overlap(boxes, boxes)
+    expect = sorted((frames[i], frames[j]) for i, j in o.tolist() if i != j)
+    key = lambda p: (p[0][0], p[1][0])
+    assert sorted(map(key, bvh.aabb_overlapping_with_self())) == expect
+
+
 def test_self_collision_detect(gpu):  # self_collision.py:5-64
     rs = np.random.RandomState(6)
     tm, bvh = _robotish(rs, 24)
