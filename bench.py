@@ -55,7 +55,7 @@ def parse():
     p.add_argument("--no-parity", action="store_true", help="skip the oracle parity checks")
     p.add_argument("--no-extra", action="store_true", help="only the headline workload")
     p.add_argument("--only", type=str, default=None,
-                   help="comma list of extra configs: hull,epa,broad,serving")
+                   help="comma list of extra configs: hull,epa,broad,serving,mpr")
     p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--serving-requests", type=int, default=50000)
     p.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -364,6 +364,47 @@ def run_serving(args, lanes):
     return stats
 
 
+def run_mpr(w, args, steps=5, warmup=1):
+    """MPR penetration (mpr.py:52-109) over the headline's pairs; parity of a
+    sample against the oracle."""
+    import torch
+    from distance3d_amd import kernels
+    from distance3d_amd.scene import collider_centers
+    t0 = time.time()
+    centers = collider_centers(w.scene)
+    dev = w.scene.to_device("cuda:0")
+    log("mpr centres and scene in %.1fs" % (time.time() - t0))
+    n = w.n_pairs
+    pa = torch.from_numpy(w.pair_a).cuda()
+    pb = torch.from_numpy(w.pair_b).cuda()
+    c = torch.from_numpy(centers).cuda()
+    st = torch.empty(n, dtype=torch.int32, device="cuda")
+    out = torch.empty((n, 7), dtype=torch.float64, device="cuda")
+    lib = kernels.lib()
+    P = kernels.ptr
+
+    def step():
+        kernels.check(lib.d3d_mpr_batch(dev.c, P(pa), P(pb), P(c), n, 0.0001, 100, 1, P(st), P(out),
+                                        kernels.stream_ptr()), "mpr")
+
+    wall, ev = timed_steps(step, steps, warmup)
+    ms = float(ev.mean())
+    res = {"workload": "mpr on mixed_pairs (1,048,576 mixed-primitive pairs, MPR penetration)",
+           "pairs": n, "value": round(n / (ms / 1e3), 1), "unit": "pair-queries/s",
+           "ms_per_step": round(ms, 4), "intersecting_frac": float((st == 1).float().mean())}
+    if not args.no_parity:
+        from oracle import oracle
+        idx = np.random.RandomState(7).choice(n, 20000, replace=False)
+        est, eout = oracle.mpr(w.scene, w.pair_a[idx], w.pair_b[idx], centers)
+        gst = st.cpu().numpy()[idx]
+        gout = out.cpu().numpy()[idx]
+        same = np.all((gout == eout) | (np.isnan(gout) & np.isnan(eout)), axis=1)
+        res["parity"] = {"pairs_checked": len(idx), "status_equal_frac": float(np.mean(gst == est)),
+                         "bit_identical_frac": float(np.mean(same))}
+    log("mpr: %.3f ms/step" % ms)
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -426,7 +467,8 @@ def main():
         result["speedup_vs_one_pair_per_launch"] = round(head["value"] / opl["value"], 1)
         log("one pair per launch: %.3g pairs/s" % opl["value"])
     if not args.no_extra:
-        only = set(args.only.split(",")) if args.only else {"hull", "epa", "broad", "serving"}
+        only = (set(args.only.split(",")) if args.only
+                else {"hull", "epa", "broad", "serving", "mpr"})
         cfgs = {}
         if "hull" in only:
             cfgs["hull_pairs"] = run_pipeline_config(workloads.hull_pairs(), args, lanes, 20, 3,
@@ -443,6 +485,8 @@ def main():
             cfgs["broad_phase"] = run_broad(args)
         if "serving" in only:
             cfgs["serving"] = run_serving(args, lanes)
+        if "mpr" in only:
+            cfgs["mpr"] = run_mpr(w, args)
         result["configs"] = cfgs
     print(json.dumps(result), flush=True)
 

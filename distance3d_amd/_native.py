@@ -84,6 +84,7 @@ _SIGNATURES = {
     "d3d_bvh_build": [_P, _I64, _P, _I64, _P],
     "d3d_bvh_query": [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P],
     "d3d_bvh_refit": [_P, _I64, _P, _I64, _P],
+    "d3d_mpr_batch": [_P, _P, _P, _P, _I64, _D, _I32, _I32, _P, _P, _P],
     "d3d_cluster_plan": [_P, _I64, _I32, _P, _P, _P],
     "d3d_cluster_scene": [_P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I32, _I32,
                           _P, _P, _P, _P, _P, _P, _P, _P],

@@ -130,6 +130,31 @@ def gjk_intersection_batch(scene, pair_a, pair_b, tolerance=1e-10, max_iteration
     return hit, status
 
 
+def mpr_batch(scene, pair_a, pair_b, centers, tolerance=0.0001, max_iterations=100,
+              penetration=True, stream=None):
+    """MPR over pairs (d3d_mpr_batch).  ``centers``: (n_shapes, 3) collider
+    centres.  Returns (status int32 tensor: 1 intersecting, 0 separated,
+    0x100 refinement cap, 0x200 bad record; out (n, 7) float64 tensor:
+    depth, direction, contact position, NaN when separated; None when
+    ``penetration`` is false)."""
+    import torch
+    a = _dev_i32(pair_a, scene.device)
+    b = _dev_i32(pair_b, scene.device)
+    n = a.numel()
+    if b.numel() != n:
+        raise ValueError("pair_a and pair_b differ in length")
+    c = torch.as_tensor(centers, dtype=torch.float64).reshape(-1, 3).to(scene.device).contiguous()
+    if c.shape[0] != scene.n_shapes:
+        raise ValueError("centers must hold one row per collider (%d), got %d"
+                         % (scene.n_shapes, c.shape[0]))
+    status = torch.empty(n, dtype=torch.int32, device=scene.device)
+    out = torch.empty((n, 7), dtype=torch.float64, device=scene.device) if penetration else None
+    check(lib().d3d_mpr_batch(scene.c, ptr(a), ptr(b), ptr(c), n, float(tolerance),
+                              int(max_iterations), 1 if penetration else 0, ptr(status), ptr(out),
+                              stream_ptr(stream)), "d3d_mpr_batch")
+    return status, out
+
+
 class EpaResult:
     """Outputs of :func:`epa_batch` (device tensors)."""
 

@@ -269,6 +269,22 @@ int d3d_instance_pairs(const d3d_scene *scene, const int32_t *pair_a, const int3
                        int32_t *inst_i32, int32_t *inst_a, int32_t *inst_b, void *stream);
 
 /*
+ * Minkowski portal refinement for n pairs (mpr.py:20-49 mpr_intersection,
+ * 52-109 mpr_penetration, with _discover_portal / _refine_portal /
+ * _find_penetration_info, :120-393).  centers: (n_shapes, 3) f64, the
+ * collider centres (collider.center()) indexed by pair_a / pair_b.
+ * status[i]: 1 intersecting, 0 separated, 0x100 the portal refinement gave
+ * up after max_iterations expansions (the reference loops without bound),
+ * 0x200 invalid collider record.  With penetration != 0, out[7 i ..] =
+ * depth, penetration direction (3), contact position (3) for intersecting
+ * pairs (NaN otherwise); with penetration == 0 only status is written
+ * (mpr_intersection) and out may be NULL.
+ */
+int d3d_mpr_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
+                  const double *centers, int64_t n, double tolerance, int32_t max_iterations,
+                  int32_t penetration, int32_t *status, double *out, void *stream);
+
+/*
  * Broad phase: linear BVH over n AABBs and overlap queries
  * (aabb_tree.py:15-191 AabbTree, 381-403 query_overlap, 465-527
  * all_aabbs_overlap / aabb_overlap).

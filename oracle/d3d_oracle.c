@@ -12,6 +12,7 @@
  *   - EPA penetration depth                     epa.py:9-202 (robust variant, see epa_pair)
  *   - AABBs of every collider                   containment.py:6-229, colliders.py:140-646
  *   - closed-interval AABB overlap / all pairs  aabb_tree.py:465-527
+ *   - MPR intersection and penetration          mpr.py:20-393, distance/_triangle.py:12-89
  *
  * Arithmetic: every expression is written left to right in the order the
  * reference writes it and the file is compiled with -ffp-contract=off, so each
@@ -769,6 +770,249 @@ static void epa_pair(const Scene *s, int ka, int kb, const V *Y, const V *P, con
     }
 }
 
+/* ----------------------------------------------------------------- MPR -- */
+
+/* Minkowski portal refinement (mpr.py:20-393, after libccd).  Portal rows:
+ * v = a - b with a on collider 1 along d and b on collider 2 along -d
+ * (minkowski.py:23-55).  Collider centres come from the caller
+ * (collider.center(), colliders.py). */
+#define MPR_OUTSIDE -1
+#define MPR_BUILT 0
+#define MPR_ON_V1 1
+#define MPR_ON_SEGMENT 2
+#define MPR_REFINE_CAP 0x100
+
+typedef struct { V v[4], a[4], b[4]; } Portal;
+
+static void mpr_support(const Scene *s, int ka, int kb, V d, V *v, V *a, V *b) {
+    *a = support(s, ka, d);
+    *b = support(s, kb, neg(d));
+    *v = sub(*a, *b);
+}
+
+static void portal_set(Portal *P, int r, V v, V a, V b) {
+    P->v[r] = v;
+    P->a[r] = a;
+    P->b[r] = b;
+}
+
+/* _portal_direction (mpr.py:286-293) */
+static V portal_dir(const Portal *P) {
+    return norm_vector(cross(sub(P->v[2], P->v[1]), sub(P->v[3], P->v[1])));
+}
+
+/* _encapsulates_origin (mpr.py:296-299) */
+static int mpr_encapsulates(V v, V d) { return dot(v, d) > -10.0 * EPS; }
+
+/* _portal_reach_tolerance (mpr.py:302-311): min over rows 1..3 of
+ * v4.d - v_r.d, first row kept unless a later one is smaller */
+static int mpr_reach_tol(const Portal *P, V v4, V d, double tol) {
+    const double p = dot(v4, d);
+    double m = p - dot(P->v[1], d);
+    for (int r = 2; r < 4; ++r) {
+        const double t = p - dot(P->v[r], d);
+        if (t < m) m = t;
+    }
+    return m < tol + EPS;
+}
+
+/* _expand_portal (mpr.py:314-329) */
+static void mpr_expand(Portal *P, V v4, V a4, V b4) {
+    const V x = cross(v4, P->v[0]);
+    if (dot(P->v[1], x) > 0.0) {
+        if (dot(P->v[2], x) > 0.0) portal_set(P, 1, v4, a4, b4);
+        else portal_set(P, 3, v4, a4, b4);
+    } else {
+        if (dot(P->v[3], x) > 0.0) portal_set(P, 2, v4, a4, b4);
+        else portal_set(P, 1, v4, a4, b4);
+    }
+}
+
+static double vnorm(V v) { return sqrt(dot(v, v)); }
+
+/* point_to_triangle(zeros, (A, B, C)) (distance/_triangle.py:12-89): the
+ * distance from the origin and the closest point */
+static double origin_to_triangle(V A, V B, V C, V *cp) {
+    const V o = mk(0.0, 0.0, 0.0);
+    const V ab = sub(B, A), ac = sub(C, A);
+    const V ap = sub(o, A);
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
+        *cp = A;
+        return vnorm(sub(o, *cp));
+    }
+    const V bp = sub(o, B);
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) {
+        *cp = B;
+        return vnorm(sub(o, *cp));
+    }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && 0.0 <= d1 && d3 <= 0.0) {
+        const double v = d1 / (d1 - d3);
+        *cp = add(A, scl(ab, v));
+        return vnorm(sub(o, *cp));
+    }
+    const V cpt = sub(o, C);
+    const double d5 = dot(ab, cpt), d6 = dot(ac, cpt);
+    if (d6 >= 0.0 && d5 <= d6) {
+        *cp = C;
+        return vnorm(sub(o, *cp));
+    }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && 0.0 <= d2 && d6 <= 0.0) {
+        const double w = d2 / (d2 - d6);
+        *cp = add(A, scl(ac, w));
+        return vnorm(sub(o, *cp));
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && 0.0 <= d4 - d3 && d5 - d6 >= 0.0) {
+        const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        *cp = add(B, scl(sub(C, B), w));
+        return vnorm(sub(o, *cp));
+    }
+    const double denom = 1.0 / (va + vb + vc);
+    const double v = vb * denom, w = vc * denom;
+    *cp = add(add(A, scl(ab, v)), scl(ac, w));
+    return vnorm(sub(o, *cp));
+}
+
+/* _contact_position (mpr.py:378-393) */
+static V mpr_contact(const Portal *P, V d) {
+    double b[4];
+    b[0] = dot(cross(P->v[1], P->v[2]), P->v[3]);
+    b[1] = dot(cross(P->v[3], P->v[2]), P->v[0]);
+    b[2] = dot(cross(P->v[0], P->v[1]), P->v[3]);
+    b[3] = dot(cross(P->v[2], P->v[1]), P->v[0]);
+    double sum = ((b[0] + b[1]) + b[2]) + b[3];
+    if (sum < EPS) {
+        b[0] = 0.0;
+        b[1] = dot(cross(P->v[2], P->v[3]), d);
+        b[2] = dot(cross(P->v[3], P->v[1]), d);
+        b[3] = dot(cross(P->v[1], P->v[2]), d);
+        sum = ((b[0] + b[1]) + b[2]) + b[3];
+    }
+    for (int r = 0; r < 4; ++r) b[r] = b[r] / sum;
+    V p1 = scl(P->a[0], b[0]), p2 = scl(P->b[0], b[0]);
+    for (int r = 1; r < 4; ++r) {
+        p1 = add(p1, scl(P->a[r], b[r]));
+        p2 = add(p2, scl(P->b[r], b[r]));
+    }
+    return scl(add(p1, p2), 0.5);
+}
+
+/* _discover_portal (mpr.py:120-160) with _find_origin_ray (:163-177),
+ * _find_support_in_direction_of_origin_ray (:180-188),
+ * _find_support_perpendicular_to_plane_containing_origin_v01 (:191-205),
+ * _search_direction_perpendicular_to_plane_containing_v012 (:208-214) and
+ * _iterate_discover_portal (:217-235). */
+static int mpr_discover(const Scene *s, int ka, int kb, V ca, V cb, int max_iter, Portal *P) {
+    V v, a, b;
+    portal_set(P, 0, sub(ca, cb), ca, cb);
+    if (P->v[0].x == 0.0 && P->v[0].y == 0.0 && P->v[0].z == 0.0) P->v[0].x += EPS * 10.0;
+    V d = norm_vector(neg(P->v[0]));
+    mpr_support(s, ka, kb, d, &v, &a, &b);
+    portal_set(P, 1, v, a, b);
+    const int v1_zero = v.x == 0.0 && v.y == 0.0 && v.z == 0.0;
+    if (!v1_zero && dot(v, d) < EPS) return MPR_OUTSIDE;
+    d = cross(P->v[0], P->v[1]);
+    if (dot(d, d) < EPS) return v1_zero ? MPR_ON_V1 : MPR_ON_SEGMENT;
+    d = norm_vector(d);
+    mpr_support(s, ka, kb, d, &v, &a, &b);
+    portal_set(P, 2, v, a, b);
+    if (dot(v, d) < EPS) return MPR_OUTSIDE;
+    d = norm_vector(cross(sub(P->v[1], P->v[0]), sub(P->v[2], P->v[0])));
+    if (dot(d, P->v[0]) > 0.0) {
+        const V tv = P->v[1], ta = P->a[1], tb = P->b[1];
+        portal_set(P, 1, P->v[2], P->a[2], P->b[2]);
+        portal_set(P, 2, tv, ta, tb);
+        d = scl(d, -1.0);
+    }
+    int n = 3, it = 0;
+    while (n < 4) {
+        mpr_support(s, ka, kb, d, &v, &a, &b);
+        portal_set(P, 3, v, a, b);
+        if (dot(v, d) < EPS) return MPR_OUTSIDE;
+        int cont = 0;
+        if (dot(cross(P->v[1], P->v[3]), P->v[0]) < EPS) {
+            portal_set(P, 2, P->v[3], P->a[3], P->b[3]);
+            cont = 1;
+        }
+        if (!cont && dot(cross(P->v[3], P->v[2]), P->v[0]) < EPS) {
+            portal_set(P, 1, P->v[3], P->a[3], P->b[3]);
+            cont = 1;
+        }
+        if (cont) d = norm_vector(cross(sub(P->v[1], P->v[0]), sub(P->v[2], P->v[0])));
+        else n = 4;
+        ++it;
+        if (it >= max_iter) break;
+    }
+    return MPR_BUILT;
+}
+
+/* _refine_portal (mpr.py:253-283).  The reference loops without bound; here
+ * the loop gives up after max_iter expansions and reports MPR_REFINE_CAP. */
+static int mpr_refine(const Scene *s, int ka, int kb, double tol, int max_iter, Portal *P) {
+    for (int it = 0;; ++it) {
+        const V d = portal_dir(P);
+        if (mpr_encapsulates(P->v[1], d)) return 1;
+        if (it >= max_iter) return MPR_REFINE_CAP;
+        V v, a, b;
+        mpr_support(s, ka, kb, d, &v, &a, &b);
+        if (!mpr_encapsulates(v, d) || mpr_reach_tol(P, v, d, tol)) return 0;
+        mpr_expand(P, v, a, b);
+    }
+}
+
+/* mpr_penetration (mpr.py:52-109) / mpr_intersection (:20-49).  status: 0
+ * separated, 1 intersecting (| MPR_REFINE_CAP when the refinement gave up);
+ * out: depth, direction (3), contact position (3), NaN when separated. */
+static int mpr_pair(const Scene *s, int ka, int kb, V ca, V cb, double tol, int max_iter,
+                    int penetration, double out[7]) {
+    for (int k = 0; k < 7; ++k) out[k] = NAN;
+    Portal P;
+    const int res = mpr_discover(s, ka, kb, ca, cb, max_iter, &P);
+    if (res == MPR_OUTSIDE) return 0;
+    if (res == MPR_ON_V1) {
+        /* _find_penetration_touch (mpr.py:361-367) */
+        const V pos = scl(add(P.a[1], P.b[1]), 0.5);
+        out[0] = 0.0;
+        out[1] = 0.0; out[2] = 0.0; out[3] = 0.0;
+        out[4] = pos.x; out[5] = pos.y; out[6] = pos.z;
+        return 1;
+    }
+    if (res == MPR_ON_SEGMENT) {
+        /* _find_penetration_segment (mpr.py:370-378) */
+        const V pos = scl(add(P.a[1], P.b[1]), 0.5);
+        const V dir = norm_vector(P.v[1]);
+        out[0] = vnorm(P.v[1]);
+        out[1] = dir.x; out[2] = dir.y; out[3] = dir.z;
+        out[4] = pos.x; out[5] = pos.y; out[6] = pos.z;
+        return 1;
+    }
+    const int hit = mpr_refine(s, ka, kb, tol, max_iter, &P);
+    if (hit != 1) return hit == MPR_REFINE_CAP ? MPR_REFINE_CAP : 0;
+    if (!penetration) return 1;
+    /* _find_penetration_info (mpr.py:332-346), _penetration_info (:349-358) */
+    for (int it = 0;; ++it) {
+        const V d = portal_dir(&P);
+        V v, a, b;
+        mpr_support(s, ka, kb, d, &v, &a, &b);
+        if (mpr_reach_tol(&P, v, d, tol) || it > max_iter) {
+            V cp;
+            const double depth = origin_to_triangle(P.v[1], P.v[2], P.v[3], &cp);
+            V dir = fabs(depth) < EPS ? mk(0.0, 0.0, 0.0) : cp;
+            dir = norm_vector(dir);
+            const V pos = mpr_contact(&P, portal_dir(&P));
+            out[0] = depth;
+            out[1] = dir.x; out[2] = dir.y; out[3] = dir.z;
+            out[4] = pos.x; out[5] = pos.y; out[6] = pos.z;
+            return 1;
+        }
+        mpr_expand(&P, v, a, b);
+    }
+}
+
 /* --------------------------------------------------------------- AABBs -- */
 
 /* collider.aabb() as (3,2) [axis][min,max], containment.py:6-229 */
@@ -1005,6 +1249,20 @@ void orc_epa(const double *f, const int32_t *i, const float *vx, const float *vy
 }
 
 /* Sub-routines exposed for the closed-form / brute-force pin tests. */
+/* Batched MPR (mpr_pair); centers: (n_shapes, 3) collider centres. */
+void orc_mpr(const double *f, const int32_t *i, const float *vx, const float *vy,
+             const float *vz, const double *v64, const int32_t *pa, const int32_t *pb,
+             const double *centers, int64_t n, double tol, int max_iter, int penetration,
+             int32_t *status, double *out) {
+    Scene s = mkscene(f, i, vx, vy, vz, v64);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t k = 0; k < n; ++k) {
+        const double *A = centers + 3 * (int64_t)pa[k], *B = centers + 3 * (int64_t)pb[k];
+        status[k] = mpr_pair(&s, pa[k], pb[k], mk(A[0], A[1], A[2]), mk(B[0], B[1], B[2]), tol,
+                             max_iter, penetration, out + 7 * k);
+    }
+}
+
 void orc_closest_point_simplex(const double *pts, int n, double *out, int32_t *set) {
     V Y[4];
     for (int k = 0; k < n; ++k) Y[k] = mk(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);

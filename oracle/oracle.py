@@ -119,6 +119,23 @@ def epa(scene, pair_a, pair_b, simplex, n_points, max_iterations=64, max_faces=1
             "iterations": iters}
 
 
+def mpr(scene, pair_a, pair_b, centers, tolerance=0.0001, max_iterations=100, penetration=True):
+    """MPR per pair (mpr.py:20-109).  ``centers``: (n_shapes, 3) collider
+    centres.  Returns (status, out): status 1 intersecting, 0 separated,
+    0x100 refinement cap; out (n, 7) depth, direction, contact position
+    (NaN when separated)."""
+    a = np.ascontiguousarray(pair_a, dtype=np.int32)
+    b = np.ascontiguousarray(pair_b, dtype=np.int32)
+    c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+    n = len(a)
+    status = np.empty(n, dtype=np.int32)
+    out = np.empty((n, 7))
+    lib().orc_mpr(*_scene_args(scene), _p(a), _p(b), _p(c), ctypes.c_int64(n),
+                  ctypes.c_double(tolerance), ctypes.c_int(max_iterations),
+                  ctypes.c_int(1 if penetration else 0), _p(status), _p(out))
+    return status, out
+
+
 def closest_point_simplex(points):
     p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
     out = np.empty(3)
