@@ -424,7 +424,7 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
                 const float4 *__restrict__ qx = reinterpret_cast<const float4 *>(px + j0);
                 const float4 *__restrict__ qy = reinterpret_cast<const float4 *>(py + j0);
                 const float4 *__restrict__ qz = reinterpret_cast<const float4 *>(pz + j0);
-#pragma unroll 1
+#pragma unroll 2
                 for (int c = 0; c < K / 4; ++c) {
                     const float4 x = qx[c], y = qy[c], z = qz[c];
                     const int j = j0 + 4 * c;
