@@ -242,14 +242,25 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
     const int64_t *__restrict__ n_dev, const int32_t *__restrict__ work,
     const int32_t *__restrict__ work_off, const double *__restrict__ simplex,
     const int32_t *__restrict__ npts, int max_iter, int max_faces, double eps,
-    double *__restrict__ out, int32_t *__restrict__ status_out, int32_t *__restrict__ iters_out) {
+    double *__restrict__ out, int32_t *__restrict__ status_out, int32_t *__restrict__ iters_out,
+    unsigned long long *__restrict__ next) {
     __shared__ EpaLds L;
     const int lane = threadIdx.x;
     const int64_t count = n_dev ? *n_dev : n;
     if (work && work_off) work += *work_off;
     for (int k = lane; k < (int)(sizeof(L.eb) / sizeof(L.eb[0])); k += kEpaBlock) L.eb[k] = 0u;
     __syncthreads();
-    for (int64_t item = blockIdx.x; item < count; item += gridDim.x) {
+    // items: claimed one at a time from `next` (pairs differ widely in cost,
+    // so a fixed stride leaves a long tail), or a fixed stride without it
+    auto claim = [&](int64_t prev) -> int64_t {
+        if (!next) return prev < 0 ? (int64_t)blockIdx.x : prev + gridDim.x;
+        unsigned long long v = 0;
+        if (lane == 0) v = atomicAdd(next, 1ull);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+        return (int64_t)(((unsigned long long)hi << 32) | lo);
+    };
+    for (int64_t item = claim(-1); item < count; item = claim(item)) {
         const int64_t pid = work ? (int64_t)work[item] : item;
         Shape SA, SB;  // validation only; supports reload the records by index
         const bool okA = load_checked(s, pa[pid], SA);
@@ -577,7 +588,8 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
 int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n_max,
                const int64_t *n_dev, const int32_t *work, const int32_t *work_off,
                const double *simplex, const int32_t *npts, int max_iter, int max_faces,
-               double eps, double *out, int32_t *status, int32_t *iters, hipStream_t stream) {
+               double eps, double *out, int32_t *status, int32_t *iters,
+               unsigned long long *next, hipStream_t stream) {
     if (n_max <= 0) return D3D_OK;
     int64_t grid = n_max;
     if (n_dev) {
@@ -591,7 +603,7 @@ int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t
     }
     hipLaunchKernelGGL(epa_kernel, dim3((unsigned)grid), dim3(kEpaBlock), 0, stream, s, pa, pb, n_max,
                        n_dev, work, work_off, simplex, npts, max_iter, max_faces, eps, out, status,
-                       iters);
+                       iters, next);
     D3D_CHECK_LAUNCH("epa_kernel");
     return D3D_OK;
 }
@@ -614,5 +626,5 @@ extern "C" int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, cons
     D3D_CHECK_ARG(max_faces >= 4 && max_faces <= kEpaMaxFaces, "max_faces must be in [4, 128]");
     return launch_epa(*scene, pair_a, pair_b, n, nullptr, nullptr, nullptr, simplex, n_points,
                       (int)max_iterations, (int)max_faces, epsilon, out, status, iterations,
-                      (hipStream_t)stream);
+                      nullptr, (hipStream_t)stream);
 }

@@ -297,7 +297,25 @@ struct GjkOut {
     double *simplex;
     int32_t *npts;
     bool simplex_hits_only;  // write simplex / npts only for intersecting pairs
+    // GJK -> EPA (all null without EPA): intersecting pairs are appended to
+    // epa_work (epa_count counts them); the others get EPA status -1 and a
+    // zero output row
+    int32_t *epa_work;
+    unsigned long long *epa_count;
+    double *epa_out;
+    int32_t *epa_status;
 };
+
+__device__ __forceinline__ void epa_route(int64_t pid, bool intersecting, const GjkOut &o) {
+    if (!o.epa_work) return;
+    if (intersecting) {
+        o.epa_work[atomicAdd(o.epa_count, 1ull)] = (int32_t)pid;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 13; ++k) o.epa_out[13 * pid + k] = 0.0;
+        o.epa_status[pid] = -1;
+    }
+}
 
 // calculate_closest_points (_gjk_jolt.py:667-687)
 __device__ __forceinline__ void closest_points(const Simplex &S, const PQ &pq, v3 &a, v3 &b) {
@@ -326,6 +344,7 @@ __device__ __forceinline__ void closest_points(const Simplex &S, const PQ &pq, v
 __device__ __forceinline__ void gjk_write(const GjkState &g, const PQ &pq, int64_t pid,
                                           double sanity, const GjkOut &o) {
     if (o.iters) o.iters[pid] = g.it;
+    epa_route(pid, g.state == D3D_INTERSECT, o);
     Simplex S = g.S;
     load_y(S, pq);
     const bool want_s = !o.simplex_hits_only || g.state == D3D_INTERSECT;
@@ -381,6 +400,7 @@ __device__ __forceinline__ void gjk_write_bad(int64_t pid, const GjkOut &o) {
     o.status[pid] = kBadInput;
     if (o.iters) o.iters[pid] = 0;
     if (o.npts) o.npts[pid] = 0;
+    epa_route(pid, false, o);
 }
 
 // ------------------------------------------------------- group supports
@@ -599,32 +619,10 @@ struct EpaHook {
     int32_t *status;
 };
 
-__global__ void zero_u64_kernel(unsigned long long *c) { *c = 0ull; }
-
-// Intersecting pairs (GJK status INTERSECT) are appended to `work` with one
-// atomic per wavefront; the others get EPA status -1 and zero outputs.
-__global__ __launch_bounds__(kQBlock) void epa_compact_kernel(const int32_t *__restrict__ status,
-                                                              int64_t n, int32_t *__restrict__ work,
-                                                              unsigned long long *__restrict__ count,
-                                                              double *__restrict__ epa_out,
-                                                              int32_t *__restrict__ epa_status) {
-    const int64_t i = (int64_t)blockIdx.x * kQBlock + threadIdx.x;
-    const bool valid = i < n;
-    const bool hit = valid && (status[i] & 0xFF) == D3D_INTERSECT;
-    const unsigned long long m = __ballot(hit);
-    if (hit) {
-        const int lane = lane_id();
-        const int first = __ffsll((long long)m) - 1;
-        unsigned long long base = 0;
-        if (lane == first) base = atomicAdd(count, (unsigned long long)__popcll(m));
-        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)base, first);
-        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(base >> 32), first);
-        work[(((unsigned long long)hi << 32) | lo) + __popcll(m & lanemask_lt(lane))] = (int32_t)i;
-    } else if (valid) {
-#pragma unroll
-        for (int k = 0; k < 13; ++k) epa_out[13 * i + k] = 0.0;
-        epa_status[i] = -1;
-    }
+// the EPA list counter and the EPA claim cursor
+__global__ void zero_u64_kernel(unsigned long long *c) {
+    c[0] = 0ull;
+    c[1] = 0ull;
 }
 
 static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n,
@@ -652,21 +650,19 @@ static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, 
         if (c == 0 && G > 4) G = 4;
         rp.lanes_per_pair[c] = G;
     }
+    if (epa) {
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, epa->count);
+        D3D_CHECK_LAUNCH("zero_u64_kernel");
+    }
     // resident at occupancy 3 (4 wavefronts per block)
     const int64_t grid = (int64_t)cu_count() * 3;
     hipLaunchKernelGGL(gjk_all_kernel, dim3((unsigned)grid), dim3(kQBlock), 0, stream, s, pa, pb,
                        w.order, w.cls_begin, w.cls_count, w.cls_cursor, rp, o);
     D3D_CHECK_LAUNCH("gjk_all_kernel");
-    if (epa) {
-        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, epa->count);
-        D3D_CHECK_LAUNCH("zero_u64_kernel");
-        hipLaunchKernelGGL(epa_compact_kernel, dim3(grid_for(n, kQBlock)), dim3(kQBlock), 0, stream,
-                           o.status, n, epa->work, epa->count, epa->out, epa->status);
-        D3D_CHECK_LAUNCH("epa_compact_kernel");
+    if (epa)
         return launch_epa(s, pa, pb, n, (const int64_t *)epa->count, epa->work, nullptr, o.simplex,
                           o.npts, epa->max_iter, epa->max_faces, epa->eps, epa->out, epa->status,
-                          nullptr, stream);
-    }
+                          nullptr, epa->count + 1, stream);
     return D3D_OK;
 }
 
@@ -764,7 +760,18 @@ static int gjk_distance_impl(const d3d_scene *scene, const int32_t *pair_a, cons
     } else {
         if (int rc = workspace_for(stream, n, &ws)) return rc;
     }
-    GjkOut o{dist, cp_a, cp_b, status, iterations, simplex, n_points, epa != nullptr};
+    GjkOut o{dist,
+             cp_a,
+             cp_b,
+             status,
+             iterations,
+             simplex,
+             n_points,
+             epa != nullptr,
+             epa ? epa->work : nullptr,
+             epa ? epa->count : nullptr,
+             epa ? epa->out : nullptr,
+             epa ? epa->status : nullptr};
     return launch_all(*scene, pair_a, pair_b, n, prm, o, ws, (hipStream_t)stream, epa);
 }
 
