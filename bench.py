@@ -55,7 +55,7 @@ def parse():
     p.add_argument("--no-parity", action="store_true", help="skip the oracle parity checks")
     p.add_argument("--no-extra", action="store_true", help="only the headline workload")
     p.add_argument("--only", type=str, default=None,
-                   help="comma list of extra configs: hull,epa,broad,serving,mpr")
+                   help="comma list of extra configs: hull,epa,broad,serving,mpr,tet")
     p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--serving-requests", type=int, default=50000)
     p.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -405,6 +405,56 @@ def run_mpr(w, args, steps=5, warmup=1):
     return res
 
 
+def run_tet(args, steps=5, warmup=1):
+    """Hydroelastic tetrahedron-pair contact (contact plane and polygon) for
+    1,048,576 random pairs between two bodies of 100,000 tetrahedra each."""
+    import torch
+    from distance3d_amd import hydroelastic_contact as hc, kernels
+    rs = np.random.RandomState(10)
+    t0 = time.time()
+    n, m = 100000, 1 << 20
+    T = [rs.uniform(-0.6, 0.6, (n, 1, 3)) + rs.randn(n, 4, 3) * 0.5 for _ in range(2)]
+    E = [rs.uniform(0.0, 1.0, (n, 4)) for _ in range(2)]
+    X = [hc.barycentric_transforms(t) for t in T]
+    pairs = np.stack([rs.randint(0, n, m), rs.randint(0, n, m)], 1).astype(np.int32)
+    log("tet workload in %.1fs" % (time.time() - t0))
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    T1, T2, e1, e2, X1, X2, pr = map(d, (T[0], T[1], E[0], E[1], X[0], X[1], pairs))
+    st = torch.empty(m, dtype=torch.int32, device="cuda")
+    planes = torch.empty((m, 4), dtype=torch.float64, device="cuda")
+    nv = torch.empty(m, dtype=torch.int32, device="cuda")
+    polys = torch.empty((m, kernels.TET_MAX_POLY, 3), dtype=torch.float64, device="cuda")
+    ws = torch.empty(int(kernels.lib().d3d_tet_pairs_workspace_bytes(m)), dtype=torch.uint8,
+                     device="cuda")
+    lib = kernels.lib()
+    P = kernels.ptr
+
+    def step():
+        kernels.check(lib.d3d_tet_pairs_batch(P(T1), P(e1), P(X1), P(T2), P(e2), P(X2), P(pr), m,
+                                              1.0, 1.0, P(st), P(planes), P(nv), P(polys), P(ws),
+                                              ws.numel(), kernels.stream_ptr()), "tet")
+
+    wall, ev = timed_steps(step, steps, warmup)
+    ms = float(ev.mean())
+    res = {"workload": "tetrahedron pairs (1,048,576 random pairs, two bodies of 100,000 "
+                       "tetrahedra): contact plane and polygon",
+           "pairs": m, "value": round(m / (ms / 1e3), 1), "unit": "pair-queries/s",
+           "ms_per_step": round(ms, 4), "intersecting_frac": float((st & 1).float().mean())}
+    if not args.no_parity:
+        from oracle import oracle
+        idx = np.random.RandomState(11).choice(m, 20000, replace=False)
+        want = oracle.tet_pairs(T[0], E[0], X[0], T[1], E[1], X[1], pairs[idx])
+        got = [t.cpu().numpy()[idx] for t in (st, planes, nv, polys)]
+        res["parity"] = {"pairs_checked": len(idx),
+                         "status_equal_frac": float(np.mean(got[0] == want[0])),
+                         "plane_bit_identical_frac": float(np.mean(np.all(got[1] == want[1], 1))),
+                         "polygon_bit_identical_frac": float(np.mean(np.all(
+                             (got[3] == want[3]) | (np.isnan(got[3]) & np.isnan(want[3])),
+                             axis=(1, 2))))}
+    log("tet: %.3f ms/step" % ms)
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -468,7 +518,7 @@ def main():
         log("one pair per launch: %.3g pairs/s" % opl["value"])
     if not args.no_extra:
         only = (set(args.only.split(",")) if args.only
-                else {"hull", "epa", "broad", "serving", "mpr"})
+                else {"hull", "epa", "broad", "serving", "mpr", "tet"})
         cfgs = {}
         if "hull" in only:
             cfgs["hull_pairs"] = run_pipeline_config(workloads.hull_pairs(), args, lanes, 20, 3,
@@ -487,6 +537,8 @@ def main():
             cfgs["serving"] = run_serving(args, lanes)
         if "mpr" in only:
             cfgs["mpr"] = run_mpr(w, args)
+        if "tet" in only:
+            cfgs["tet_pairs"] = run_tet(args)
         result["configs"] = cfgs
     print(json.dumps(result), flush=True)
 

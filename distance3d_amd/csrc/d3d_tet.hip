@@ -1,0 +1,270 @@
+// Hydroelastic contact of tetrahedron pairs (hydroelastic_contact/
+// _tetrahedron_intersection.py:7-423, _halfplanes.py:8-64): for each pair the
+// contact plane (where the Young's-modulus-weighted pressure fields agree) and
+// the contact polygon (that plane clipped by both tetrahedra).
+//
+// One lane per pair, FP64 throughout, in the order of the reference's numpy
+// expressions.  The eight half-planes stay in registers (every loop over them
+// is unrolled with compile-time indices); the up to 23 polygon points of a
+// pair go to a per-pair workspace row, their angles and the sort order to
+// LDS, so nothing is indexed dynamically in registers.
+#include "d3d_device.hpp"
+#include "d3d_runtime.hpp"
+
+namespace d3d {
+
+constexpr int kTetBlock = 64;
+constexpr int kTetMaxPoly = 24;  // 3 x 8 half-planes, as the reference reserves
+constexpr int kTetSame = 0x10;
+constexpr int kTetCuts = 0x20;  // the plane cuts both tetrahedra
+constexpr int kTetOverflow = 0x100;
+
+// numpy's pairwise sum (n <= 128) of a[0], a[st], ...: a plain loop below 8
+// terms, else eight running sums combined pairwise, then the tail
+__device__ __forceinline__ double np_sum(const double *a, int n, int st) {
+    if (n < 8) {
+        double r = 0.0;
+        for (int i = 0; i < n; ++i) r += a[i * st];
+        return r;
+    }
+    double r0 = a[0], r1 = a[st], r2 = a[2 * st], r3 = a[3 * st];
+    double r4 = a[4 * st], r5 = a[5 * st], r6 = a[6 * st], r7 = a[7 * st];
+    int i = 8;
+    for (; i < n - (n % 8); i += 8) {
+        r0 += a[i * st]; r1 += a[(i + 1) * st]; r2 += a[(i + 2) * st]; r3 += a[(i + 3) * st];
+        r4 += a[(i + 4) * st]; r5 += a[(i + 5) * st]; r6 += a[(i + 6) * st]; r7 += a[(i + 7) * st];
+    }
+    double res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+    for (; i < n; ++i) res += a[i * st];
+    return res;
+}
+
+// plane_basis_from_normal (utils.py:78-122)
+__device__ __forceinline__ void plane_basis(v3 n, v3 &x, v3 &y) {
+    if (fabs(n.x) >= fabs(n.y)) {
+        const double length = sqrt(n.x * n.x + n.z * n.z);
+        x = mk(-n.z / length, 0.0, n.x / length);
+        y = mk(n.y * x.z, n.z * x.x - n.x * x.z, -n.y * x.x);
+    } else {
+        const double length = sqrt(n.y * n.y + n.z * n.z);
+        x = mk(0.0, n.z / length, -n.y / length);
+        y = mk(n.y * x.z - n.z * x.y, -n.x * x.z, n.x * x.y);
+    }
+}
+
+__global__ __launch_bounds__(kTetBlock) void tet_pair_kernel(
+    const double *__restrict__ T1, const double *__restrict__ e1, const double *__restrict__ X1,
+    const double *__restrict__ T2, const double *__restrict__ e2, const double *__restrict__ X2,
+    const int32_t *__restrict__ pairs, int64_t m, double E1, double E2,
+    int32_t *__restrict__ status_out, double *__restrict__ planes, int32_t *__restrict__ nv_out,
+    double *__restrict__ polys, double *__restrict__ work) {
+    __shared__ double ang_s[kTetMaxPoly][kTetBlock];
+    __shared__ unsigned char ord_s[kTetMaxPoly][kTetBlock];
+    const int lane = threadIdx.x;
+    const int64_t k = (int64_t)blockIdx.x * kTetBlock + lane;
+    if (k >= m) return;
+    const int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
+    const double *t1 = T1 + 12 * i, *t2 = T2 + 12 * j;
+    const double *x1 = X1 + 16 * i, *x2 = X2 + 16 * j;
+    const double *p1 = e1 + 4 * i, *p2 = e2 + 4 * j;
+    double *plane = planes + 4 * k;
+    double *poly = polys + 3 * kTetMaxPoly * k;
+    double *pts = work + 2 * kTetMaxPoly * k;
+    int status = 0, nv = 0;
+
+    // contact_plane (_tetrahedron_intersection.py:176-229)
+    double w1[4], w2[4], h[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        w1[r] = p1[r] * E1;
+        w2[r] = p2[r] * E2;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double a = ((w1[0] * x1[c] + w1[1] * x1[4 + c]) + w1[2] * x1[8 + c]) + w1[3] * x1[12 + c];
+        const double b = ((w2[0] * x2[c] + w2[1] * x2[4 + c]) + w2[2] * x2[8 + c]) + w2[3] * x2[12 + c];
+        h[c] = a - b;
+    }
+    const double norm = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    bool same = norm == 0.0;
+    if (!same) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) h[c] = h[c] / norm;
+        h[3] = h[3] * -1.0;
+        same = fabs(h[3]) < 10.0 * kEps;
+    }
+    if (same) {
+        // _handle_same_tetrahedron (:142-159)
+        double sum = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum += p2[r];
+        double wt[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) wt[r] = p2[r] / sum;
+        const v3 pp = mk(((wt[0] * t2[0] + wt[1] * t2[3]) + wt[2] * t2[6]) + wt[3] * t2[9],
+                         ((wt[0] * t2[1] + wt[1] * t2[4]) + wt[2] * t2[7]) + wt[3] * t2[10],
+                         ((wt[0] * t2[2] + wt[1] * t2[5]) + wt[2] * t2[8]) + wt[3] * t2[11]);
+        const double d = sqrt(dot(pp, pp));
+        const v3 n = d > 0.0 ? mk(pp.x / d, pp.y / d, pp.z / d) : mk(0.0, 0.0, 1.0);
+        h[0] = n.x; h[1] = n.y; h[2] = n.z; h[3] = d;
+        for (int q = 0; q < 3; ++q) {
+            poly[3 * q] = pp.x; poly[3 * q + 1] = pp.y; poly[3 * q + 2] = pp.z;
+        }
+        nv = 3;
+        status = 1 | kTetSame;
+    } else {
+        const v3 n = mk(h[0], h[1], h[2]);
+        const double d = h[3];
+        // check_tetrahedra_intersect_contact_plane (:232-258), tolerance 1e-6
+        bool crosses = true;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const double *T = t == 0 ? t1 : t2;
+            double mn = 0.0, mx = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double dist = (T[3 * r] * n.x + T[3 * r + 1] * n.y + T[3 * r + 2] * n.z) - d;
+                if (r == 0 || dist < mn) mn = dist;
+                if (r == 0 || dist > mx) mx = dist;
+            }
+            if (!(mn < -1e-6 && mx > 1e-6)) crosses = false;
+        }
+        if (crosses) {
+            status = kTetCuts;
+            // compute_contact_polygon (:363-423), make_halfplanes (:261-297);
+            // faces whose projection vanishes are dropped
+            const v3 pp = n * d;
+            v3 ax, ay;
+            plane_basis(n, ax, ay);
+            double hp[8][4];
+#pragma unroll
+            for (int g = 0; g < 8; ++g) hp[g][0] = hp[g][1] = hp[g][2] = hp[g][3] = 0.0;
+            int nh = 0;
+#pragma unroll
+            for (int f = 0; f < 8; ++f) {
+                const double *X = f < 4 ? x1 + 4 * f : x2 + 4 * (f - 4);
+                const v3 fn = mk(X[0], X[1], X[2]);
+                const double n0 = dot(fn, ax), n1 = dot(fn, ay);
+                const double ds = -X[3] - dot(fn, pp);
+                const double nn = sqrt(n0 * n0 + n1 * n1);
+                const bool keep = nn > kEps;
+                // row nh of the compacted list: written to every slot f' >= nh
+                // it could land in, by selects (f' is a constant)
+#pragma unroll
+                for (int g = 0; g <= f; ++g) {
+                    const bool w = keep && g == nh;
+                    hp[g][0] = w ? n0 * ds / (nn * nn) : hp[g][0];
+                    hp[g][1] = w ? n1 * ds / (nn * nn) : hp[g][1];
+                    hp[g][2] = w ? n1 : hp[g][2];
+                    hp[g][3] = w ? -n0 : hp[g][3];
+                }
+                nh += keep ? 1 : 0;
+            }
+            // intersect_halfplanes (_halfplanes.py:29-64)
+            int cnt = 0;
+            bool overflow = false;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+#pragma unroll
+                for (int b = a + 1; b < 8; ++b) {
+                    if (b >= nh) continue;
+                    const double denom = hp[a][2] * hp[b][3] - hp[a][3] * hp[b][2];
+                    if (fabs(denom) < kEps) continue;
+                    const double q0 = hp[b][0] - hp[a][0], q1 = hp[b][1] - hp[a][1];
+                    const double t = (q0 * hp[b][3] - q1 * hp[b][2]) / denom;
+                    const double px = hp[a][0] + hp[a][2] * t, py = hp[a][1] + hp[a][3] * t;
+                    bool valid = true;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        if (c == a || c == b || c >= nh || !valid) continue;
+                        const double r0 = px - hp[c][0], r1 = py - hp[c][1];
+                        if (hp[c][2] * r1 - hp[c][3] * r0 < -kEps) valid = false;
+                    }
+                    if (valid) {
+                        if (cnt + 1 >= 3 * nh) {
+                            overflow = true;
+                        } else {
+                            pts[2 * cnt] = px;
+                            pts[2 * cnt + 1] = py;
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+            if (overflow) {
+                status = kTetOverflow | kTetCuts;
+            } else if (cnt >= 3) {
+                // order_points (:300-318): by atan2 about the mean point,
+                // stable insertion sort
+                const double cx = np_sum(pts, cnt, 2) / cnt, cy = np_sum(pts + 1, cnt, 2) / cnt;
+                for (int q = 0; q < cnt; ++q) {
+                    ang_s[q][lane] = atan2(pts[2 * q + 1] - cy, pts[2 * q] - cx);
+                    ord_s[q][lane] = (unsigned char)q;
+                }
+                for (int q = 1; q < cnt; ++q) {
+                    const int o = ord_s[q][lane];
+                    const double ao = ang_s[o][lane];
+                    int r = q - 1;
+                    while (r >= 0 && ang_s[ord_s[r][lane]][lane] > ao) {
+                        ord_s[r + 1][lane] = ord_s[r][lane];
+                        --r;
+                    }
+                    ord_s[r + 1][lane] = (unsigned char)o;
+                }
+                // filter_unique_points (:321-340): dropped within 10 eps of the
+                // previous point in angular order; project_polygon_to_3d (:343-360)
+                int nu = 0;
+                for (int q = 0; q < cnt; ++q) {
+                    const int o = ord_s[q][lane];
+                    const double ux = pts[2 * o], uy = pts[2 * o + 1];
+                    if (q > 0) {
+                        const int pr = ord_s[q - 1][lane];
+                        const double dx = ux - pts[2 * pr], dy = uy - pts[2 * pr + 1];
+                        if (!(sqrt(dx * dx + dy * dy) > 10.0 * kEps)) continue;
+                    }
+                    poly[3 * nu] = (ux * ax.x + uy * ay.x) + pp.x;
+                    poly[3 * nu + 1] = (ux * ax.y + uy * ay.y) + pp.y;
+                    poly[3 * nu + 2] = (ux * ax.z + uy * ay.z) + pp.z;
+                    ++nu;
+                }
+                if (nu >= 3) {
+                    nv = nu;
+                    status = 1 | kTetCuts;
+                }
+            }
+        }
+    }
+    plane[0] = h[0]; plane[1] = h[1]; plane[2] = h[2]; plane[3] = h[3];
+    for (int q = 3 * nv; q < 3 * kTetMaxPoly; ++q) poly[q] = NAN;
+    status_out[k] = status;
+    nv_out[k] = nv;
+}
+
+}  // namespace d3d
+
+using namespace d3d;
+
+extern "C" int64_t d3d_tet_pairs_workspace_bytes(int64_t m) {
+    if (m < 0) return -1;
+    return m * 2 * kTetMaxPoly * (int64_t)sizeof(double);
+}
+
+extern "C" int d3d_tet_pairs_batch(const double *T1, const double *e1, const double *X1,
+                                   const double *T2, const double *e2, const double *X2,
+                                   const int32_t *pairs, int64_t m, double youngs_modulus1,
+                                   double youngs_modulus2, int32_t *status, double *planes,
+                                   int32_t *n_vertices, double *polygons, void *workspace,
+                                   int64_t workspace_bytes, void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(m >= 0, "m must be >= 0");
+    if (m == 0) return D3D_OK;
+    D3D_CHECK_ARG(T1 && e1 && X1 && T2 && e2 && X2 && pairs && status && planes && n_vertices &&
+                      polygons && workspace,
+                  "NULL buffer");
+    D3D_CHECK_ARG(workspace_bytes >= d3d_tet_pairs_workspace_bytes(m), "workspace too small");
+    hipLaunchKernelGGL(tet_pair_kernel, dim3(grid_for(m, kTetBlock)), dim3(kTetBlock), 0,
+                       (hipStream_t)stream, T1, e1, X1, T2, e2, X2, pairs, m, youngs_modulus1,
+                       youngs_modulus2, status, planes, n_vertices, polygons, (double *)workspace);
+    D3D_CHECK_LAUNCH("tet_pair_kernel");
+    return D3D_OK;
+}

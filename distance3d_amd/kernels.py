@@ -155,6 +155,43 @@ def mpr_batch(scene, pair_a, pair_b, centers, tolerance=0.0001, max_iterations=1
     return status, out
 
 
+TET_MAX_POLY = 24
+
+
+def tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, youngs_modulus1=1.0, youngs_modulus2=1.0,
+                    stream=None):
+    """Contact plane and polygon of tetrahedron pairs (d3d_tet_pairs_batch).
+    T (n,4,3), e (n,4), X (n,4,4) per body, pairs (m,2); tensors or arrays.
+    Returns device tensors (status (m,), planes (m,4), n_vertices (m,),
+    polygons (m,24,3))."""
+    import torch
+
+    def dev(a, shape):
+        t = torch.as_tensor(a, dtype=torch.float64)
+        return (t if t.is_cuda else t.cuda()).reshape(shape).contiguous()
+
+    T1, T2 = dev(T1, (-1, 4, 3)), dev(T2, (-1, 4, 3))
+    e1, e2 = dev(e1, (-1, 4)), dev(e2, (-1, 4))
+    X1, X2 = dev(X1, (-1, 4, 4)), dev(X2, (-1, 4, 4))
+    pr = torch.as_tensor(pairs, dtype=torch.int32)
+    pr = (pr if pr.is_cuda else pr.cuda()).reshape(-1, 2).contiguous()
+    m = pr.shape[0]
+    if m and (int(pr[:, 0].max()) >= T1.shape[0] or int(pr[:, 1].max()) >= T2.shape[0]
+              or int(pr.min()) < 0):
+        raise ValueError("pair index out of range")
+    status = torch.empty(m, dtype=torch.int32, device="cuda")
+    planes = torch.empty((m, 4), dtype=torch.float64, device="cuda")
+    nv = torch.empty(m, dtype=torch.int32, device="cuda")
+    polys = torch.empty((m, TET_MAX_POLY, 3), dtype=torch.float64, device="cuda")
+    ws = torch.empty(max(int(_native.load_library().d3d_tet_pairs_workspace_bytes(m)), 1),
+                     dtype=torch.uint8, device="cuda")
+    check(lib().d3d_tet_pairs_batch(ptr(T1), ptr(e1), ptr(X1), ptr(T2), ptr(e2), ptr(X2), ptr(pr),
+                                    m, float(youngs_modulus1), float(youngs_modulus2), ptr(status),
+                                    ptr(planes), ptr(nv), ptr(polys), ptr(ws), ws.numel(),
+                                    stream_ptr(stream)), "d3d_tet_pairs_batch")
+    return status, planes, nv, polys
+
+
 class EpaResult:
     """Outputs of :func:`epa_batch` (device tensors)."""
 

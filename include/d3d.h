@@ -285,6 +285,29 @@ int d3d_mpr_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *
                   int32_t penetration, int32_t *status, double *out, void *stream);
 
 /*
+ * Hydroelastic contact of m tetrahedron pairs (hydroelastic_contact/
+ * _tetrahedron_intersection.py:7-85 intersect_tetrahedron_pairs, 89-139
+ * intersect_tetrahedron_pair and the contact plane / polygon routines it
+ * calls, _halfplanes.py:8-64).  Body k has n_k tetrahedra: points T_k
+ * (n_k,4,3) f64, pressure potentials e_k (n_k,4), barycentric transforms X_k
+ * (n_k,4,4) (barycentric_transforms, _barycentric_transform.py:4-9); pairs
+ * (m,2) int32 index (body 1, body 2).  Per pair: status (bit 0 the
+ * tetrahedra intersect, 0x10 the two potential fields are the same plane
+ * family -- the reference's same-tetrahedron case --, 0x20 the contact plane
+ * cuts both tetrahedra (a polygon was computed), 0x100 more half-plane
+ * intersection points than the reference reserves, where it asserts), the
+ * contact plane in Hesse normal form planes (m,4), and the contact polygon
+ * in counter-clockwise order n_vertices (m), polygons (m,24,3) (NaN past
+ * n_vertices).  workspace: d3d_tet_pairs_workspace_bytes(m) device bytes.
+ */
+int64_t d3d_tet_pairs_workspace_bytes(int64_t m);
+int d3d_tet_pairs_batch(const double *T1, const double *e1, const double *X1, const double *T2,
+                        const double *e2, const double *X2, const int32_t *pairs, int64_t m,
+                        double youngs_modulus1, double youngs_modulus2, int32_t *status,
+                        double *planes, int32_t *n_vertices, double *polygons, void *workspace,
+                        int64_t workspace_bytes, void *stream);
+
+/*
  * Broad phase: linear BVH over n AABBs and overlap queries
  * (aabb_tree.py:15-191 AabbTree, 381-403 query_overlap, 465-527
  * all_aabbs_overlap / aabb_overlap).

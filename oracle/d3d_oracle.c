@@ -13,6 +13,8 @@
  *   - AABBs of every collider                   containment.py:6-229, colliders.py:140-646
  *   - closed-interval AABB overlap / all pairs  aabb_tree.py:465-527
  *   - MPR intersection and penetration          mpr.py:20-393, distance/_triangle.py:12-89
+ *   - tetrahedron-pair contact planes/polygons  hydroelastic_contact/_tetrahedron_intersection.py,
+ *                                               _halfplanes.py, utils.py:78-122
  *
  * Arithmetic: every expression is written left to right in the order the
  * reference writes it and the file is compiled with -ffp-contract=off, so each
@@ -1013,6 +1015,206 @@ static int mpr_pair(const Scene *s, int ka, int kb, V ca, V cb, double tol, int 
     }
 }
 
+/* ---------------------------------------------- tetrahedron contact -- */
+
+/* Hydroelastic contact of tetrahedron pairs (hydroelastic_contact/
+ * _tetrahedron_intersection.py:7-423, _halfplanes.py:8-64): the contact plane
+ * where the Young's-modulus-weighted potentials are equal, and the contact
+ * polygon (the plane clipped by both tetrahedra) from half-plane
+ * intersection in the plane. */
+#define TET_MAX_POLY 24
+#define TET_SAME 0x10
+#define TET_CUTS 0x20 /* the plane cuts both tetrahedra (a polygon was computed) */
+#define TET_OVERFLOW 0x100
+
+/* numpy's pairwise sum of n doubles at stride st (numpy/core/src/umath/
+ * loops_utils.h pairwise_sum for n <= 128: plain loop below 8, else eight
+ * running sums combined as ((0+1)+(2+3))+((4+5)+(6+7)) then the tail). */
+static double np_sum(const double *a, int n, int st) {
+    if (n < 8) {
+        double r = 0.0;
+        for (int i = 0; i < n; ++i) r += a[i * st];
+        return r;
+    }
+    double r[8];
+    for (int j = 0; j < 8; ++j) r[j] = a[j * st];
+    int i;
+    for (i = 8; i < n - (n % 8); i += 8)
+        for (int j = 0; j < 8; ++j) r[j] += a[(i + j) * st];
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += a[i * st];
+    return res;
+}
+
+/* plane_basis_from_normal (utils.py:78-122) */
+static void plane_basis(V n, V *x, V *y) {
+    if (fabs(n.x) >= fabs(n.y)) {
+        const double length = sqrt(n.x * n.x + n.z * n.z);
+        *x = mk(-n.z / length, 0.0, n.x / length);
+        *y = mk(n.y * x->z, n.z * x->x - n.x * x->z, -n.y * x->x);
+    } else {
+        const double length = sqrt(n.y * n.y + n.z * n.z);
+        *x = mk(0.0, n.z / length, -n.y / length);
+        *y = mk(n.y * x->z - n.z * x->y, -n.x * x->z, n.x * x->y);
+    }
+}
+
+static double cross2d(const double *a, const double *b) { return a[0] * b[1] - a[1] * b[0]; }
+
+/* intersect_halfplanes (_halfplanes.py:29-64) with intersect_two_halfplanes
+ * (:17-23) and point_outside_of_halfplane (:26-28).  hp: n rows (p, pq).
+ * Returns the point count, or -1 when more than 3 n - 1 points are found (the
+ * reference asserts). */
+static int intersect_halfplanes(const double (*hp)[4], int n, double (*pts)[2]) {
+    int cnt = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            const double denom = cross2d(hp[i] + 2, hp[j] + 2);
+            if (fabs(denom) < EPS) continue;
+            const double q[2] = {hp[j][0] - hp[i][0], hp[j][1] - hp[i][1]};
+            const double t = cross2d(q, hp[j] + 2) / denom;
+            const double p[2] = {hp[i][0] + hp[i][2] * t, hp[i][1] + hp[i][3] * t};
+            int valid = 1;
+            for (int k = 0; k < n; ++k) {
+                if (k == i || k == j) continue;
+                const double r[2] = {p[0] - hp[k][0], p[1] - hp[k][1]};
+                if (cross2d(hp[k] + 2, r) < -EPS) {
+                    valid = 0;
+                    break;
+                }
+            }
+            if (valid) {
+                if (cnt + 1 >= 3 * n) return -1;
+                pts[cnt][0] = p[0];
+                pts[cnt][1] = p[1];
+                ++cnt;
+            }
+        }
+    return cnt;
+}
+
+/* intersect_tetrahedron_pair (_tetrahedron_intersection.py:89-139) for
+ * tetrahedra T1, T2 (4 x 3), potentials e1, e2 (4), barycentric transforms
+ * X1, X2 (4 x 4, row major).  plane: Hesse normal form (4); poly: up to
+ * TET_MAX_POLY points (3); returns status (bit 0: intersection). */
+static int tet_pair(const double *T1, const double *e1, const double *X1, const double *T2,
+                    const double *e2, const double *X2, double E1, double E2, double plane[4],
+                    int *nv, double *poly) {
+    *nv = 0;
+    /* contact_plane (:176-229) */
+    double w1[4], w2[4], h[4];
+    for (int r = 0; r < 4; ++r) {
+        w1[r] = e1[r] * E1;
+        w2[r] = e2[r] * E2;
+    }
+    for (int c = 0; c < 4; ++c) {
+        const double a = ((w1[0] * X1[c] + w1[1] * X1[4 + c]) + w1[2] * X1[8 + c]) + w1[3] * X1[12 + c];
+        const double b = ((w2[0] * X2[c] + w2[1] * X2[4 + c]) + w2[2] * X2[8 + c]) + w2[3] * X2[12 + c];
+        h[c] = a - b;
+    }
+    const double norm = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    int same = norm == 0.0;
+    if (!same) {
+        for (int c = 0; c < 4; ++c) h[c] = h[c] / norm;
+        h[3] = h[3] * -1.0;
+        same = fabs(h[3]) < 10.0 * EPS;
+    }
+    if (same) {
+        /* _handle_same_tetrahedron (:142-159) */
+        double sum = 0.0;
+        for (int r = 0; r < 4; ++r) sum += e2[r];
+        double wt[4];
+        for (int r = 0; r < 4; ++r) wt[r] = e2[r] / sum;
+        V pp;
+        pp.x = ((wt[0] * T2[0] + wt[1] * T2[3]) + wt[2] * T2[6]) + wt[3] * T2[9];
+        pp.y = ((wt[0] * T2[1] + wt[1] * T2[4]) + wt[2] * T2[7]) + wt[3] * T2[10];
+        pp.z = ((wt[0] * T2[2] + wt[1] * T2[5]) + wt[2] * T2[8]) + wt[3] * T2[11];
+        const double d = sqrt(dot(pp, pp));
+        const V n = d > 0.0 ? mk(pp.x / d, pp.y / d, pp.z / d) : mk(0.0, 0.0, 1.0);
+        plane[0] = n.x; plane[1] = n.y; plane[2] = n.z; plane[3] = d;
+        for (int k = 0; k < 3; ++k) {
+            poly[3 * k] = pp.x; poly[3 * k + 1] = pp.y; poly[3 * k + 2] = pp.z;
+        }
+        *nv = 3;
+        return 1 | TET_SAME;
+    }
+    for (int c = 0; c < 4; ++c) plane[c] = h[c];
+    const V n = mk(h[0], h[1], h[2]);
+    const double d = h[3];
+    /* check_tetrahedra_intersect_contact_plane (:232-258), tolerance 1e-6 */
+    const double tol = 1e-6;
+    for (int t = 0; t < 2; ++t) {
+        const double *T = t == 0 ? T1 : T2;
+        double mn = 0.0, mx = 0.0;
+        for (int r = 0; r < 4; ++r) {
+            const double dist = (T[3 * r] * n.x + T[3 * r + 1] * n.y + T[3 * r + 2] * n.z) - d;
+            if (r == 0 || dist < mn) mn = dist;
+            if (r == 0 || dist > mx) mx = dist;
+        }
+        if (!(mn < -tol && mx > tol)) return 0;
+    }
+    /* compute_contact_polygon (:363-423) with make_halfplanes (:261-297) */
+    const V pp = scl(n, d);
+    V ax, ay;
+    plane_basis(n, &ax, &ay);
+    double hp[8][4];
+    int nh = 0;
+    for (int i = 0; i < 8; ++i) {
+        const double *X = i < 4 ? X1 + 4 * i : X2 + 4 * (i - 4);
+        const V fn = mk(X[0], X[1], X[2]);
+        const double n2[2] = {dot(fn, ax), dot(fn, ay)};
+        const double ds = -X[3] - dot(fn, pp);
+        const double nn = sqrt(n2[0] * n2[0] + n2[1] * n2[1]);
+        if (nn > EPS) {
+            hp[nh][0] = n2[0] * ds / (nn * nn);
+            hp[nh][1] = n2[1] * ds / (nn * nn);
+            hp[nh][2] = n2[1];
+            hp[nh][3] = -n2[0];
+            ++nh;
+        }
+    }
+    double pts[24][2];
+    const int np = intersect_halfplanes(hp, nh, pts);
+    if (np < 0) return TET_OVERFLOW | TET_CUTS;
+    if (np < 3) return TET_CUTS;
+    /* order_points (:300-318): by atan2 around the mean point, stable */
+    const double cx = np_sum(&pts[0][0], np, 2) / np, cy = np_sum(&pts[0][1], np, 2) / np;
+    double ang[24];
+    int ord[24];
+    for (int k = 0; k < np; ++k) {
+        ang[k] = atan2(pts[k][1] - cy, pts[k][0] - cx);
+        ord[k] = k;
+    }
+    for (int k = 1; k < np; ++k) {
+        const int o = ord[k];
+        int q = k - 1;
+        while (q >= 0 && ang[ord[q]] > ang[o]) {
+            ord[q + 1] = ord[q];
+            --q;
+        }
+        ord[q + 1] = o;
+    }
+    /* filter_unique_points (:321-340): a point is kept unless it is within
+     * 10 eps of the point before it in angular order (kept or not); then
+     * project_polygon_to_3d (:343-360) */
+    int nu = 0;
+    for (int k = 0; k < np; ++k) {
+        const double *p = pts[ord[k]];
+        if (k > 0) {
+            const double *q = pts[ord[k - 1]];
+            const double dx = p[0] - q[0], dy = p[1] - q[1];
+            if (!(sqrt(dx * dx + dy * dy) > 10.0 * EPS)) continue;
+        }
+        poly[3 * nu] = (p[0] * ax.x + p[1] * ay.x) + pp.x;
+        poly[3 * nu + 1] = (p[0] * ax.y + p[1] * ay.y) + pp.y;
+        poly[3 * nu + 2] = (p[0] * ax.z + p[1] * ay.z) + pp.z;
+        ++nu;
+    }
+    if (nu < 3) return TET_CUTS;
+    *nv = nu;
+    return 1 | TET_CUTS;
+}
+
 /* --------------------------------------------------------------- AABBs -- */
 
 /* collider.aabb() as (3,2) [axis][min,max], containment.py:6-229 */
@@ -1261,6 +1463,28 @@ void orc_mpr(const double *f, const int32_t *i, const float *vx, const float *vy
         status[k] = mpr_pair(&s, pa[k], pb[k], mk(A[0], A[1], A[2]), mk(B[0], B[1], B[2]), tol,
                              max_iter, penetration, out + 7 * k);
     }
+}
+
+/* Batched tetrahedron-pair contact (tet_pair); pairs (m, 2) index the
+ * tetrahedra of body 1 (T1 (n1,4,3), e1 (n1,4), X1 (n1,4,4)) and body 2. */
+void orc_tet_pairs(const double *T1, const double *e1, const double *X1, const double *T2,
+                   const double *e2, const double *X2, const int32_t *pairs, int64_t m,
+                   double E1, double E2, int32_t *status, double *planes, int32_t *nv,
+                   double *polys) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < m; ++k) {
+        const int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
+        int n = 0;
+        for (int c = 0; c < 3 * TET_MAX_POLY; ++c) polys[3 * TET_MAX_POLY * k + c] = NAN;
+        status[k] = tet_pair(T1 + 12 * i, e1 + 4 * i, X1 + 16 * i, T2 + 12 * j, e2 + 4 * j,
+                             X2 + 16 * j, E1, E2, planes + 4 * k, &n, polys + 3 * TET_MAX_POLY * k);
+        nv[k] = n;
+    }
+}
+
+/* intersect_halfplanes on n rows (p, pq); returns the point count or -1. */
+int orc_intersect_halfplanes(const double *hp, int n, double *pts) {
+    return intersect_halfplanes((const double (*)[4])hp, n, (double (*)[2])pts);
 }
 
 void orc_closest_point_simplex(const double *pts, int n, double *out, int32_t *set) {

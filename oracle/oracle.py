@@ -30,6 +30,7 @@ def lib():
         _lib.orc_num_threads.restype = ctypes.c_int
         _lib.orc_all_aabbs_overlap.restype = ctypes.c_int64
         _lib.orc_overlap_sorted.restype = ctypes.c_int64
+        _lib.orc_intersect_halfplanes.restype = ctypes.c_int
     return _lib
 
 
@@ -134,6 +135,41 @@ def mpr(scene, pair_a, pair_b, centers, tolerance=0.0001, max_iterations=100, pe
                   ctypes.c_double(tolerance), ctypes.c_int(max_iterations),
                   ctypes.c_int(1 if penetration else 0), _p(status), _p(out))
     return status, out
+
+
+TET_MAX_POLY = 24
+
+
+def tet_pairs(T1, e1, X1, T2, e2, X2, pairs, youngs_modulus1=1.0, youngs_modulus2=1.0):
+    """Contact plane and polygon per tetrahedron pair
+    (_tetrahedron_intersection.py:89-139).  Returns (status (m,) int32, bit 0
+    intersecting, 0x10 same-tetrahedron case, 0x20 the plane cuts both,
+    0x100 half-plane overflow;
+    planes (m, 4); n_vertices (m,); polygons (m, 24, 3), NaN past n)."""
+    c = lambda a, s: np.ascontiguousarray(a, dtype=np.float64).reshape(s)
+    T1 = c(T1, (-1, 4, 3)); T2 = c(T2, (-1, 4, 3))
+    e1 = c(e1, (-1, 4)); e2 = c(e2, (-1, 4))
+    X1 = c(X1, (-1, 4, 4)); X2 = c(X2, (-1, 4, 4))
+    pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+    m = len(pr)
+    status = np.empty(m, dtype=np.int32)
+    planes = np.empty((m, 4))
+    nv = np.empty(m, dtype=np.int32)
+    polys = np.empty((m, TET_MAX_POLY, 3))
+    lib().orc_tet_pairs(_p(T1), _p(e1), _p(X1), _p(T2), _p(e2), _p(X2), _p(pr), ctypes.c_int64(m),
+                        ctypes.c_double(youngs_modulus1), ctypes.c_double(youngs_modulus2),
+                        _p(status), _p(planes), _p(nv), _p(polys))
+    return status, planes, nv, polys
+
+
+def intersect_halfplanes(halfplanes):
+    """_halfplanes.py:29-64; raises AssertionError where the reference does."""
+    hp = np.ascontiguousarray(halfplanes, dtype=np.float64).reshape(-1, 4)
+    pts = np.empty((max(3 * len(hp), 1), 2))
+    n = lib().orc_intersect_halfplanes(_p(hp), ctypes.c_int(len(hp)), _p(pts))
+    if n < 0:
+        raise AssertionError("more half-plane intersections than the reference reserves")
+    return pts[:n]
 
 
 def closest_point_simplex(points):

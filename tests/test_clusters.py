@@ -1,8 +1,8 @@
 """Clustered device layout of hull vertex sets (d3d_cluster_scene, host code):
 the layout is a per-hull permutation that keeps records sharing a range
 together, clustered hulls are aligned, every cluster's sphere contains its
-vertices, and the FP32 bound the device evaluates (c.d + r|d| as one FMA
-chain, |d| rounded up) is never below the exact score of a cluster vertex."""
+vertices, and the FP32 bounds the device evaluates (c.d +- r|d| as one FMA
+chain, |d| rounded up) bracket the exact score of every cluster vertex."""
 import numpy as np
 import pytest
 
@@ -81,11 +81,15 @@ def test_cluster_bound_never_below_exact_scores():
         for l in dirs:
             d = l.astype(np.float32)
             nl = np.float32(np.sqrt(l @ l) * (1.0 + 2.0 ** -19))
-            ub = _fma32(C[:, 3], np.float32(nl),
-                        _fma32(C[:, 2], d[2], _fma32(C[:, 1], d[1], (C[:, 0] * d[0]).astype(np.float32))))
+            dot = _fma32(C[:, 2], d[2], _fma32(C[:, 1], d[1], (C[:, 0] * d[0]).astype(np.float32)))
+            ub = _fma32(C[:, 3], np.float32(nl), dot)
+            lb = _fma32(-C[:, 3], np.float32(nl), dot)
             exact = V @ l.astype(np.longdouble)
             per_cluster = np.array([exact[q * K:(q + 1) * K].max() for q in range(nc)])
+            per_cluster_min = np.array([exact[q * K:(q + 1) * K].min() for q in range(nc)])
             assert np.all(ub.astype(np.longdouble) >= per_cluster)
+            # c.d - r|d| with the same inflation is never above a vertex score
+            assert np.all(lb.astype(np.longdouble) <= per_cluster_min)
             checked += 1
         # containment with room to spare for the inflation
         for q in range(nc):
