@@ -500,8 +500,8 @@ def main():
                    "pairs": w.n_pairs, "seed": args.seed, "hull_vertices": "log-uniform 8..4096",
                    "lanes_per_pair": lanes or list(_native.DEFAULT_LANES)},
         "roofline": dict(head["roofline"], traffic=traffic_bytes(),
-                         kernel="gjk_epa pipeline step (sort + 8 work-class GJK kernels + "
-                                "compaction + EPA)", kernel_ms=head["event_ms_per_step"]),
+                         kernel="gjk_epa pipeline step (counting sort + gjk_all_kernel + "
+                                "epa_kernel)", kernel_ms=head["event_ms_per_step"]),
     }
     if "parity" in head:
         result["parity"] = head["parity"]
