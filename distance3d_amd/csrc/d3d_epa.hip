@@ -250,12 +250,17 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
     if (work && work_off) work += *work_off;
     for (int k = lane; k < (int)(sizeof(L.eb) / sizeof(L.eb[0])); k += kEpaBlock) L.eb[k] = 0u;
     __syncthreads();
-    // items: claimed one at a time from `next` (pairs differ widely in cost,
-    // so a fixed stride leaves a long tail), or a fixed stride without it
+    // items: each wavefront's first is its block index; later ones are
+    // claimed one at a time from `next`, past the first items (pairs differ
+    // widely in cost, so a fixed stride leaves a long tail), or follow in a
+    // fixed stride without it.  A wavefront whose first item is past the end
+    // leaves without an atomic (a small batch starts far more wavefronts than
+    // it has EPA pairs).
     auto claim = [&](int64_t prev) -> int64_t {
-        if (!next) return prev < 0 ? (int64_t)blockIdx.x : prev + gridDim.x;
+        if (prev < 0) return (int64_t)blockIdx.x;
+        if (!next) return prev + gridDim.x;
         unsigned long long v = 0;
-        if (lane == 0) v = atomicAdd(next, 1ull);
+        if (lane == 0) v = gridDim.x + atomicAdd(next, 1ull);
         const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
         const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
         return (int64_t)(((unsigned long long)hi << 32) | lo);
