@@ -92,6 +92,124 @@ __device__ __forceinline__ double wave_uniform(double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Butterfly exchange on the VALU (DPP and permlane swaps, no LDS round trip).
+// xpair<OFF>(x, a, b) leaves in (a, b) the values x of the two halves of the
+// lane's aligned 2*OFF-lane block that a reduction step at distance OFF
+// merges: for OFF <= 8, a = the lane's own x and b = x of lane ^ OFF (OFF = 1,
+// 2: quad permutes) or of the mirrored lane (OFF = 4: row_half_mirror, lane
+// 7 - i of the 8-lane block; OFF = 8: row_mirror, lane 15 - i of the row),
+// which holds the same value as lane ^ OFF once every aligned OFF-lane block
+// is uniform; for OFF = 16, 32, a = x of the lower half and b = x of the upper
+// half (permlane16/32 swap of x with itself).  Reductions therefore step OFF =
+// 1, 2, 4, ... upwards and merge (a, b) with a symmetric, exact selection
+// (max, min, argmax with index ties, top-2), after which every lane of the
+// group holds the group's result.  Every lane of the group must be active.
+template <int N> struct IntC { static constexpr int value = N; };
+
+template <int OFF>
+__device__ __forceinline__ void xpair(unsigned x, unsigned &a, unsigned &b) {
+    static_assert(OFF == 1 || OFF == 2 || OFF == 4 || OFF == 8 || OFF == 16 || OFF == 32, "");
+    if constexpr (OFF <= 8) {
+        constexpr int ctrl = OFF == 1 ? 0xB1 : OFF == 2 ? 0x4E : OFF == 4 ? 0x141 : 0x140;
+        a = x;
+        b = (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, ctrl, 0xF, 0xF, false);
+    } else if constexpr (OFF == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        a = p[0];
+        b = p[1];
+    } else {
+        const auto p = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        a = p[0];
+        b = p[1];
+    }
+}
+template <int OFF>
+__device__ __forceinline__ void xpair(int x, int &a, int &b) {
+    unsigned ua, ub;
+    xpair<OFF>((unsigned)x, ua, ub);
+    a = (int)ua;
+    b = (int)ub;
+}
+template <int OFF>
+__device__ __forceinline__ void xpair(float x, float &a, float &b) {
+    unsigned ua, ub;
+    xpair<OFF>(__float_as_uint(x), ua, ub);
+    a = __uint_as_float(ua);
+    b = __uint_as_float(ub);
+}
+template <int OFF>
+__device__ __forceinline__ void xpair(double x, double &a, double &b) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    unsigned la, lb, ha, hb;
+    xpair<OFF>((unsigned)u, la, lb);
+    xpair<OFF>((unsigned)(u >> 32), ha, hb);
+    a = __longlong_as_double((long long)(((unsigned long long)ha << 32) | la));
+    b = __longlong_as_double((long long)(((unsigned long long)hb << 32) | lb));
+}
+
+// Calls f(IntC<OFF>) for OFF = 1, 2, 4, ... < G: the steps of a G-lane
+// butterfly reduction in the order xpair requires.
+template <int G, int OFF = 1, class F>
+__device__ __forceinline__ void for_butterfly(F &&f) {
+    if constexpr (OFF < G) {
+        f(IntC<OFF>{});
+        for_butterfly<G, OFF * 2>(f);
+    }
+}
+
+// Merge of two FP32 (largest m1 at i1, largest other m2) summaries: exact,
+// symmetric in its operands; ties in m1 go to the smaller index.
+__device__ __forceinline__ void top2_merge(float am1, float am2, int ai, float bm1, float bm2, int bi,
+                                           float &m1, float &m2, int &i1) {
+    m2 = fmaxf(fmaxf(am2, bm2), fminf(am1, bm1));
+    const bool tb = (bm1 > am1) || (bm1 == am1 && bi < ai);
+    m1 = tb ? bm1 : am1;
+    i1 = tb ? bi : ai;
+}
+
+template <int G>
+__device__ __forceinline__ void group_top2(float &m1, float &m2, int &i1) {
+    for_butterfly<G>([&](auto o) {
+        constexpr int OFF = decltype(o)::value;
+        float am1, bm1, am2, bm2;
+        int ai, bi;
+        xpair<OFF>(m1, am1, bm1);
+        xpair<OFF>(m2, am2, bm2);
+        xpair<OFF>(i1, ai, bi);
+        top2_merge(am1, am2, ai, bm1, bm2, bi, m1, m2, i1);
+    });
+}
+
+// (largest value, smallest index among its ties) over the G lanes of a group.
+template <int G>
+__device__ __forceinline__ void group_argmaxf(float &v, int &k) {
+    for_butterfly<G>([&](auto o) {
+        constexpr int OFF = decltype(o)::value;
+        float av, bv;
+        int ak, bk;
+        xpair<OFF>(v, av, bv);
+        xpair<OFF>(k, ak, bk);
+        const bool tb = (bv > av) || (bv == av && bk < ak);
+        v = tb ? bv : av;
+        k = tb ? bk : ak;
+    });
+}
+
+// (smallest value, smallest index among its ties) over the G lanes of a group.
+template <int G>
+__device__ __forceinline__ void group_argmin(double &v, int &k) {
+    for_butterfly<G>([&](auto o) {
+        constexpr int OFF = decltype(o)::value;
+        double av, bv;
+        int ak, bk;
+        xpair<OFF>(v, av, bv);
+        xpair<OFF>(k, ak, bk);
+        const bool tb = (bv < av) || (bv == av && bk < ak);
+        v = tb ? bv : av;
+        k = tb ? bk : ak;
+    });
+}
+
 // ------------------------------------------------------------- colliders
 
 // Collider record as laid out in the scene (see d3d.h).
@@ -217,8 +335,11 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long m, i
 
 template <int G>
 __device__ __forceinline__ float group_maxf(float x) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off));
+    for_butterfly<G>([&](auto o) {
+        float a, b;
+        xpair<decltype(o)::value>(x, a, b);
+        x = fmaxf(a, b);
+    });
     return x;
 }
 
@@ -274,16 +395,18 @@ struct ArgmaxF64 {
     // merge over the G lanes of a group; the index (0 when nothing was taken)
     template <int G>
     __device__ __forceinline__ int group_result() {
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-            const double ob = __shfl_xor(best, off);
-            const int oi = __shfl_xor(bi, off);
-            const int oo = __shfl_xor(bo, off);
-            const bool t = (ob > best) || (ob == best && oo < bo);
-            best = t ? ob : best;
-            bi = t ? oi : bi;
-            bo = t ? oo : bo;
-        }
+        for_butterfly<G>([&](auto o) {
+            constexpr int OFF = decltype(o)::value;
+            double ab, bb;
+            int ai, bi2, ao, bo2;
+            xpair<OFF>(best, ab, bb);
+            xpair<OFF>(bi, ai, bi2);
+            xpair<OFF>(bo, ao, bo2);
+            const bool t = (bb > ab) || (bb == ab && bo2 < ao);
+            best = t ? bb : ab;
+            bi = t ? bi2 : ai;
+            bo = t ? bo2 : ao;
+        });
         return bi == 0x7fffffff ? 0 : bi;
     }
 };
@@ -396,14 +519,7 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
                 bk = k;
             }
         }
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-            const float ou = __shfl_xor(bu, off);
-            const int ok = __shfl_xor(bk, off);
-            const bool take = (ou > bu) || (ou == bu && ok < bk);
-            bu = take ? ou : bu;
-            bk = take ? ok : bk;
-        }
+        group_argmaxf<G>(bu, bk);
         const int ks = bk < nc ? bk : 0;
         // the best cluster, scanned by the group (lane sub takes slots sub, sub+G, ...)
         {
@@ -439,16 +555,7 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
             thr = fmaxf(thr, __double2float_rd((double)m1 - 2.5 * E));
         }
     }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        const float om1 = __shfl_xor(m1, off);
-        const float om2 = __shfl_xor(m2, off);
-        const int oi = __shfl_xor(i1, off);
-        m2 = fmaxf(fmaxf(m2, om2), fminf(m1, om1));
-        const bool take = (om1 > m1) || (om1 == m1 && oi < i1);
-        m1 = take ? om1 : m1;
-        i1 = take ? oi : i1;
-    }
+    group_top2<G>(m1, m2, i1);
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
 
     return hull_rescan<G>(s, H, l, sub, __double2float_rd((double)m1 - 2.5 * E));

@@ -57,16 +57,7 @@ __device__ __forceinline__ int lds_hull_argmax(const d3d_scene &s, const EpaLds 
     for (int j = lane; j < H.nv; j += kWave)
         top2(__builtin_fmaf(L.cz[off + j], dz, __builtin_fmaf(L.cy[off + j], dy, L.cx[off + j] * dx)), j,
              m1, m2, i1);
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const float om1 = __shfl_xor(m1, o);
-        const float om2 = __shfl_xor(m2, o);
-        const int oi = __shfl_xor(i1, o);
-        m2 = fmaxf(fmaxf(m2, om2), fminf(m1, om1));
-        const bool take = (om1 > m1) || (om1 == m1 && oi < i1);
-        m1 = take ? om1 : m1;
-        i1 = take ? oi : i1;
-    }
+    group_top2<kWave>(m1, m2, i1);
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
     // the FP64 rescan of hull_rescan, with the candidates found in the cache
     float thr = __double2float_rd((double)m1 - 2.5 * E);
@@ -130,14 +121,21 @@ __device__ __forceinline__ bool edge_has(const EpaLds &L, int a, int b) {
 
 __device__ __forceinline__ v3 lds_w(const EpaLds &L, int v) { return mk(L.wx[v], L.wy[v], L.wz[v]); }
 
+// Exclusive prefix sum over the wavefront (all lanes active) and its total:
+// Hillis-Steele steps 1, 2, 4, 8 inside each 16-lane row by DPP row shifts
+// (a lane shifted in from outside the row reads 0), then the row totals
+// (lanes 15, 31, 47, 63) read into scalars and added to the later rows.
 __device__ __forceinline__ int wave_excl_scan(int c, int lane, int &total) {
     int incl = c;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int t = __shfl_up(incl, off);
-        if (lane >= off) incl += t;
-    }
-    total = __shfl(incl, kWave - 1);
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xF, 0xF, false);  // row_shr:1
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xF, 0xF, false);  // row_shr:2
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xF, 0xF, false);  // row_shr:4
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xF, 0xF, false);  // row_shr:8
+    const int r0 = readlane_i32(incl, 15), r1 = readlane_i32(incl, 31);
+    const int r2 = readlane_i32(incl, 47), r3 = readlane_i32(incl, 63);
+    const int row = lane >> 4;
+    incl += (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
+    total = r0 + r1 + r2 + r3;
     return incl - c;
 }
 
@@ -373,14 +371,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                     if (d < bd) { bd = d; bk = k; }
                 }
             }
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const double od = __shfl_xor(bd, off);
-                const int ok = __shfl_xor(bk, off);
-                const bool take = (od < bd) || (od == bd && ok < bk);
-                bd = take ? od : bd;
-                bk = take ? ok : bk;
-            }
+            group_argmin<kWave>(bd, bk);
             kb = bk;
             const v3 cn = mk(L.fnx[kb], L.fny[kb], L.fnz[kb]);
             const double cdist = L.fd[kb];
@@ -457,9 +448,11 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                     }
                 }
             }
-            int tot0, tot1;
-            const int pre0 = wave_excl_scan(cnt[0], lane, tot0);
-            const int pre1 = tot0 + wave_excl_scan(cnt[1], lane, tot1);
+            // both counts (at most 3 each) in one scan, slot 1 in the high half
+            int tot;
+            const int pre = wave_excl_scan(cnt[0] | (cnt[1] << 16), lane, tot);
+            const int tot0 = tot & 0xffff, tot1 = tot >> 16;
+            const int pre0 = pre & 0xffff, pre1 = tot0 + (pre >> 16);
             const bool kept0 = (lane < nf) && !rem[0];
             const bool kept1 = (kWave + lane < nf) && !rem[1];
             const unsigned long long k0 = __ballot(kept0), k1 = __ballot(kept1);
@@ -539,14 +532,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                     if (d < bd) { bd = d; bk = k; }
                 }
             }
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const double od = __shfl_xor(bd, off);
-                const int ok = __shfl_xor(bk, off);
-                const bool take = (od < bd) || (od == bd && ok < bk);
-                bd = take ? od : bd;
-                bk = take ? ok : bk;
-            }
+            group_argmin<kWave>(bd, bk);
             kb = bk == 0x7fffffff ? 0 : bk;
             if (nf > 0) {
                 depth = L.fd[kb];

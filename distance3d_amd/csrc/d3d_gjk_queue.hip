@@ -450,7 +450,7 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
             // one atomic fetches the next chunk when it runs out.  Groups
             // ranked past the items at hand wait for the next round.
             const int c = __popcll(m);
-            const int rank = __shfl(__popcll(m & lanemask_lt(lane)), leader);
+            const int rank = __popcll(m & lanemask_lt(leader));
             const int avail = ce - cb;
             int nb = 0;
             if (avail < c) {
