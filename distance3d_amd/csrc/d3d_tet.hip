@@ -8,6 +8,7 @@
 // is unrolled with compile-time indices); the up to 23 polygon points of a
 // pair go to a per-pair workspace row, their angles and the sort order to
 // LDS, so nothing is indexed dynamically in registers.
+#include "d3d_atan2.hpp"
 #include "d3d_device.hpp"
 #include "d3d_runtime.hpp"
 
@@ -194,11 +195,11 @@ __global__ __launch_bounds__(kTetBlock) void tet_pair_kernel(
             if (overflow) {
                 status = kTetOverflow | kTetCuts;
             } else if (cnt >= 3) {
-                // order_points (:300-318): by atan2 about the mean point,
-                // stable insertion sort
+                // order_points (:300-318): by the angle about the mean point
+                // (atan2_cr, correctly rounded), stable insertion sort
                 const double cx = np_sum(pts, cnt, 2) / cnt, cy = np_sum(pts + 1, cnt, 2) / cnt;
                 for (int q = 0; q < cnt; ++q) {
-                    ang_s[q][lane] = atan2(pts[2 * q + 1] - cy, pts[2 * q] - cx);
+                    ang_s[q][lane] = atan2_cr(pts[2 * q + 1] - cy, pts[2 * q] - cx);
                     ord_s[q][lane] = (unsigned char)q;
                 }
                 for (int q = 1; q < cnt; ++q) {
@@ -240,6 +241,206 @@ __global__ __launch_bounds__(kTetBlock) void tet_pair_kernel(
     nv_out[k] = nv;
 }
 
+// LU factorisation with partial pivoting of a 4x4 matrix in the order of
+// LAPACK's unblocked getf2 (np.linalg.solve reaches it through gesv): per
+// column the first row of largest |a|, the row swap (by selects, so every
+// index stays a compile-time constant and the matrix stays in registers),
+// multipliers by the pivot's reciprocal (by division below DBL_MIN), the
+// rank-1 update of the trailing block.  False when a pivot is zero.
+__device__ __forceinline__ bool lu4(double (&A)[4][4], int (&piv)[4]) {
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int p = j;
+        double best = fabs(A[j][j]);
+#pragma unroll
+        for (int i = j + 1; i < 4; ++i) {
+            const double v = fabs(A[i][j]);
+            const bool t = v > best;
+            best = t ? v : best;
+            p = t ? i : p;
+        }
+        piv[j] = p;
+#pragma unroll
+        for (int i = j + 1; i < 4; ++i) {
+            const bool sw = p == i;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double t = A[j][c];
+                A[j][c] = sw ? A[i][c] : t;
+                A[i][c] = sw ? t : A[i][c];
+            }
+        }
+        const double d = A[j][j];
+        if (d != 0.0) {
+            const bool recip = fabs(d) >= 2.2250738585072014e-308;
+            const double r = 1.0 / d;
+#pragma unroll
+            for (int i = j + 1; i < 4; ++i) A[i][j] = recip ? A[i][j] * r : A[i][j] / d;
+        } else {
+            ok = false;
+        }
+#pragma unroll
+        for (int i = j + 1; i < 4; ++i)
+#pragma unroll
+            for (int c = j + 1; c < 4; ++c) A[i][c] = A[i][c] - A[i][j] * A[j][c];
+    }
+    return ok;
+}
+
+// getrs for one right-hand side: the row swaps in order, the unit lower and
+// the upper triangular solves (column-oriented; a zero entry skips its
+// column, as the reference triangular solve does).
+__device__ __forceinline__ void lu4_solve(const double (&A)[4][4], const int (&piv)[4],
+                                          double (&b)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = j + 1; i < 4; ++i) {
+            const bool sw = piv[j] == i;
+            const double t = b[j];
+            b[j] = sw ? b[i] : t;
+            b[i] = sw ? t : b[i];
+        }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool nz = b[k] != 0.0;
+#pragma unroll
+        for (int i = k + 1; i < 4; ++i) b[i] = nz ? b[i] - b[k] * A[i][k] : b[i];
+    }
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+        const bool nz = b[k] != 0.0;
+        b[k] = nz ? b[k] / A[k][k] : b[k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) b[i] = nz ? b[i] - b[k] * A[i][k] : b[i];
+    }
+}
+
+// [T^T; 1 1 1 1] of a tetrahedron (T: 4 points, row-major): its inverse maps
+// (x, 1) to barycentric coordinates (_barycentric_transform.py:4-9,
+// _forces.py:94).
+__device__ __forceinline__ void tet_matrix(const double *__restrict__ T, double (&A)[4][4]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        A[0][c] = T[3 * c];
+        A[1][c] = T[3 * c + 1];
+        A[2][c] = T[3 * c + 2];
+        A[3][c] = 1.0;
+    }
+}
+
+// Per tetrahedron: AABB [axis][min, max] (_mesh_processing.py:4-20, NaN
+// propagates as np.min/np.max do) and the barycentric transform, the inverse
+// of [T^T; 1] (the reference's pseudo-inverse equals it for a non-degenerate
+// tetrahedron); flags = 1 for a zero pivot.
+__global__ __launch_bounds__(kTetBlock) void tet_prep_kernel(const double *__restrict__ T, int64_t n,
+                                                             double *__restrict__ aabbs,
+                                                             double *__restrict__ X,
+                                                             int32_t *__restrict__ flags) {
+    const int64_t k = (int64_t)blockIdx.x * kTetBlock + threadIdx.x;
+    if (k >= n) return;
+    const double *t = T + 12 * k;
+    if (aabbs) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double mn = t[a], mx = t[a];
+#pragma unroll
+            for (int r = 1; r < 4; ++r) {
+                const double v = t[3 * r + a];
+                mn = (v < mn || v != v) ? v : mn;
+                mx = (v > mx || v != v) ? v : mx;
+            }
+            aabbs[6 * k + 2 * a] = mn;
+            aabbs[6 * k + 2 * a + 1] = mx;
+        }
+    }
+    if (X || flags) {
+        double A[4][4];
+        int piv[4];
+        tet_matrix(t, A);
+        const bool ok = lu4(A, piv);
+        if (flags) flags[k] = ok ? 0 : 1;
+        if (X) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                double b[4] = {0.0, 0.0, 0.0, 0.0};
+                b[c] = 1.0;
+                lu4_solve(A, piv, b);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[16 * k + 4 * r + c] = b[r];
+            }
+        }
+    }
+}
+
+// compute_contact_force (_forces.py:52-111) for each pair with status bit 0
+// (zero rows for the others), from the first body's tetrahedron and
+// potentials: the polygon as the fan (0, q, q + 1), at most the reference's
+// six triangles (its TRIANGLES table is the fan of an 8-gon, _forces.py:
+// 47-48); per triangle the pressure at the centroid (barycentric coordinates
+// by the LU solve, weighted by potentials x Young's modulus) times the area.
+// out row: centre (3), force (3), area.
+__global__ __launch_bounds__(kTetBlock) void tet_force_kernel(
+    const double *__restrict__ T1, const double *__restrict__ e1, const int32_t *__restrict__ pairs,
+    const int32_t *__restrict__ status, const double *__restrict__ planes,
+    const int32_t *__restrict__ n_vertices, const double *__restrict__ polys, int64_t m, double E1,
+    double *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * kTetBlock + threadIdx.x;
+    if (k >= m) return;
+    double *o = out + 7 * k;
+    if (!(status[k] & 1)) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) o[c] = 0.0;
+        return;
+    }
+    const int64_t i = pairs[2 * k];
+    const double *p1 = e1 + 4 * i;
+    const double *poly = polys + 3 * kTetMaxPoly * k;
+    double A[4][4];
+    int piv[4];
+    tet_matrix(T1 + 12 * i, A);
+    lu4(A, piv);
+    double we[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) we[r] = p1[r] * E1;
+    const int ntri = min(n_vertices[k] - 2, 6);
+    const v3 v0 = mk(poly[0], poly[1], poly[2]);
+    double tf = 0.0, ta = 0.0;
+    v3 com = mk(0.0, 0.0, 0.0);
+    for (int q = 1; q <= ntri; ++q) {
+        const v3 v1 = mk(poly[3 * q], poly[3 * q + 1], poly[3 * q + 2]);
+        const v3 v2 = mk(poly[3 * q + 3], poly[3 * q + 4], poly[3 * q + 5]);
+        const v3 c3 = mk(((v0.x + v1.x) + v2.x) / 3.0, ((v0.y + v1.y) + v2.y) / 3.0,
+                         ((v0.z + v1.z) + v2.z) / 3.0);
+        double b[4] = {c3.x, c3.y, c3.z, 1.0};
+        lu4_solve(A, piv, b);
+        double pr = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pr += b[r] * we[r];
+        const v3 cr = cross(v1 - v0, v2 - v0);
+        const double area = 0.5 * sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+        tf += pr * area;
+        ta += area;
+        com = mk(com.x + area * c3.x, com.y + area * c3.y, com.z + area * c3.z);
+    }
+    if (ta > 0.0) {
+        o[0] = com.x / ta; o[1] = com.y / ta; o[2] = com.z / ta;
+    } else {
+        o[0] = v0.x; o[1] = v0.y; o[2] = v0.z;
+    }
+    const double *pl = planes + 4 * k;
+    o[3] = tf * pl[0]; o[4] = tf * pl[1]; o[5] = tf * pl[2];
+    o[6] = ta;
+}
+
+__global__ __launch_bounds__(kTetBlock) void angle_kernel(const double *__restrict__ y,
+                                                          const double *__restrict__ x, int64_t n,
+                                                          double *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * kTetBlock + threadIdx.x;
+    if (k < n) out[k] = atan2_cr(y[k], x[k]);
+}
+
 }  // namespace d3d
 
 using namespace d3d;
@@ -266,5 +467,46 @@ extern "C" int d3d_tet_pairs_batch(const double *T1, const double *e1, const dou
                        (hipStream_t)stream, T1, e1, X1, T2, e2, X2, pairs, m, youngs_modulus1,
                        youngs_modulus2, status, planes, n_vertices, polygons, (double *)workspace);
     D3D_CHECK_LAUNCH("tet_pair_kernel");
+    return D3D_OK;
+}
+
+extern "C" int d3d_tet_mesh_prep(const double *T, int64_t n, double *aabbs, double *X,
+                                 int32_t *flags, void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(T && (aabbs || X || flags), "NULL buffer");
+    hipLaunchKernelGGL(tet_prep_kernel, dim3(grid_for(n, kTetBlock)), dim3(kTetBlock), 0,
+                       (hipStream_t)stream, T, n, aabbs, X, flags);
+    D3D_CHECK_LAUNCH("tet_prep_kernel");
+    return D3D_OK;
+}
+
+extern "C" int d3d_tet_contact_forces(const double *T1, const double *e1, const int32_t *pairs,
+                                      const int32_t *status, const double *planes,
+                                      const int32_t *n_vertices, const double *polygons,
+                                      int64_t m, double youngs_modulus1, double *out,
+                                      void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(m >= 0, "m must be >= 0");
+    if (m == 0) return D3D_OK;
+    D3D_CHECK_ARG(T1 && e1 && pairs && status && planes && n_vertices && polygons && out,
+                  "NULL buffer");
+    hipLaunchKernelGGL(tet_force_kernel, dim3(grid_for(m, kTetBlock)), dim3(kTetBlock), 0,
+                       (hipStream_t)stream, T1, e1, pairs, status, planes, n_vertices, polygons, m,
+                       youngs_modulus1, out);
+    D3D_CHECK_LAUNCH("tet_force_kernel");
+    return D3D_OK;
+}
+
+extern "C" int d3d_polygon_angles(const double *y, const double *x, int64_t n, double *out,
+                                  void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return D3D_OK;
+    D3D_CHECK_ARG(y && x && out, "NULL buffer");
+    hipLaunchKernelGGL(angle_kernel, dim3(grid_for(n, kTetBlock)), dim3(kTetBlock), 0,
+                       (hipStream_t)stream, y, x, n, out);
+    D3D_CHECK_LAUNCH("angle_kernel");
     return D3D_OK;
 }

@@ -192,6 +192,73 @@ def tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, youngs_modulus1=1.0, youngs_m
     return status, planes, nv, polys
 
 
+def _dev64(a, shape):
+    import torch
+    t = torch.as_tensor(a, dtype=torch.float64)
+    return (t if t.is_cuda else t.cuda()).reshape(shape).contiguous()
+
+
+def _dev32(a, shape):
+    import torch
+    t = torch.as_tensor(a, dtype=torch.int32)
+    return (t if t.is_cuda else t.cuda()).reshape(shape).contiguous()
+
+
+def polygon_angles(y, x, stream=None):
+    """atan2(y, x) rounded correctly, elementwise (d3d_polygon_angles): the
+    angle the contact-polygon ordering uses.  Returns a device tensor."""
+    import torch
+    y = _dev64(y, (-1,))
+    x = _dev64(x, (-1,))
+    if y.shape != x.shape:
+        raise ValueError("y and x differ in length")
+    out = torch.empty_like(y)
+    check(lib().d3d_polygon_angles(ptr(y), ptr(x), y.shape[0], ptr(out), stream_ptr(stream)),
+          "d3d_polygon_angles")
+    return out
+
+
+def tet_mesh_prep(T, aabbs=True, transforms=True, stream=None):
+    """AABBs (n,3,2) and barycentric transforms (n,4,4) of tetrahedra T
+    (n,4,3) (d3d_tet_mesh_prep).  Returns device tensors (aabbs or None, X or
+    None, flags (n,) int32: 1 for a degenerate tetrahedron)."""
+    import torch
+    T = _dev64(T, (-1, 4, 3))
+    n = T.shape[0]
+    ab = torch.empty((n, 3, 2), dtype=torch.float64, device="cuda") if aabbs else None
+    X = torch.empty((n, 4, 4), dtype=torch.float64, device="cuda") if transforms else None
+    flags = torch.empty(n, dtype=torch.int32, device="cuda")
+    check(lib().d3d_tet_mesh_prep(ptr(T), n, ptr(ab) if ab is not None else None,
+                                  ptr(X) if X is not None else None, ptr(flags),
+                                  stream_ptr(stream)), "d3d_tet_mesh_prep")
+    return ab, X, flags
+
+
+def tet_contact_forces(T1, e1, pairs, status, planes, n_vertices, polygons, youngs_modulus1=1.0,
+                       stream=None):
+    """Centre, force and area of each intersecting pair's contact polygon
+    (d3d_tet_contact_forces); inputs as tet_pairs_batch takes and returns
+    them.  Returns a device tensor (m, 7): centre (3), force (3), area."""
+    import torch
+    T1 = _dev64(T1, (-1, 4, 3))
+    e1 = _dev64(e1, (-1, 4))
+    pr = _dev32(pairs, (-1, 2))
+    m = pr.shape[0]
+    st = _dev32(status, (m,))
+    pl = _dev64(planes, (m, 4))
+    nv = _dev32(n_vertices, (m,))
+    po = _dev64(polygons, (m, TET_MAX_POLY, 3))
+    if m and (int(pr[:, 0].max()) >= T1.shape[0] or int(pr[:, 0].min()) < 0):
+        raise ValueError("pair index out of range")
+    if m and int(nv.max()) > TET_MAX_POLY:
+        raise ValueError("polygon vertex count out of range")
+    out = torch.empty((m, 7), dtype=torch.float64, device="cuda")
+    check(lib().d3d_tet_contact_forces(ptr(T1), ptr(e1), ptr(pr), ptr(st), ptr(pl), ptr(nv), ptr(po),
+                                       m, float(youngs_modulus1), ptr(out), stream_ptr(stream)),
+          "d3d_tet_contact_forces")
+    return out
+
+
 class EpaResult:
     """Outputs of :func:`epa_batch` (device tensors)."""
 

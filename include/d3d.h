@@ -308,6 +308,43 @@ int d3d_tet_pairs_batch(const double *T1, const double *e1, const double *X1, co
                         int64_t workspace_bytes, void *stream);
 
 /*
+ * The angle the contact-polygon ordering sorts by (order_points,
+ * hydroelastic_contact/_tetrahedron_intersection.py:300-318, np.arctan2):
+ * out[k] = atan2(y[k], x[k]) rounded correctly to FP64 (C99 special cases).
+ * np.arctan2's last bit varies with the host's numpy build and C library;
+ * this is the one fixed choice, shared with the pair kernel.
+ */
+int d3d_polygon_angles(const double *y, const double *x, int64_t n, double *out, void *stream);
+
+/*
+ * Per-tetrahedron preparation of a tetrahedral mesh (T (n,4,3) f64): its
+ * AABBs (n,3,2) [axis][min,max] (tetrahedral_mesh_aabbs,
+ * hydroelastic_contact/_mesh_processing.py:4-20) and barycentric transforms
+ * X (n,4,4), the inverse of [T^T; 1 1 1 1] by LU with partial pivoting
+ * (barycentric_transforms, _barycentric_transform.py:4-9, which takes the
+ * pseudo-inverse: the same matrix for a non-degenerate tetrahedron, equal
+ * to rounding); flags (n) = 1 where a pivot is zero (degenerate
+ * tetrahedron).  Any of aabbs, X, flags may be NULL (not computed).
+ */
+int d3d_tet_mesh_prep(const double *T, int64_t n, double *aabbs, double *X, int32_t *flags,
+                      void *stream);
+
+/*
+ * Contact forces of tetrahedron pairs (compute_contact_force,
+ * hydroelastic_contact/_forces.py:52-111, batched as contact_surface_forces
+ * :6-24 does): for each pair whose status (from d3d_tet_pairs_batch) has bit
+ * 0, from the first body's tetrahedron T1[pairs[k,0]], potentials
+ * e1[pairs[k,0]] and Young's modulus and the pair's plane and polygon
+ * (planes, n_vertices, polygons as d3d_tet_pairs_batch writes them): out
+ * (m,7) = polygon centre (3), force vector (3), area; zero rows for the other
+ * pairs.
+ */
+int d3d_tet_contact_forces(const double *T1, const double *e1, const int32_t *pairs,
+                           const int32_t *status, const double *planes,
+                           const int32_t *n_vertices, const double *polygons, int64_t m,
+                           double youngs_modulus1, double *out, void *stream);
+
+/*
  * Broad phase: linear BVH over n AABBs and overlap queries
  * (aabb_tree.py:15-191 AabbTree, 381-403 query_overlap, 465-527
  * all_aabbs_overlap / aabb_overlap).

@@ -1027,6 +1027,117 @@ static int mpr_pair(const Scene *s, int ka, int kb, V ca, V cb, double tol, int 
 #define TET_CUTS 0x20 /* the plane cuts both tetrahedra (a polygon was computed) */
 #define TET_OVERFLOW 0x100
 
+/* atan2 rounded correctly to FP64 (up to cases within about 2^-100 relative
+ * of a rounding midpoint).  order_points sorts by np.arctan2, whose last-bit
+ * rounding depends on the host's numpy build and C library (neither rounds
+ * every case correctly), and near-duplicate polygon points are ordered by
+ * that last bit; the correctly rounded angle is the one fixed choice.
+ * Double-double arithmetic (error-free sums and products by fma), one final
+ * rounding: r = min/max of |y|, |x|; atan(r) = atan(c) + atan(u) with
+ * c = k/16 nearest r, u = (r - c)/(1 + r c) (|u| <= 1/32), atan(c) from a
+ * table, atan(u) by its odd series to u^21; then pi/2 - . when |y| > |x|,
+ * pi - . when x < 0, and the sign of y.  C99 special cases. */
+typedef struct { double hi, lo; } DD;
+static DD dd_two_sum(double a, double b) {
+    double s = a + b, bb = s - a;
+    DD r = {s, (a - (s - bb)) + (b - bb)};
+    return r;
+}
+static DD dd_fast(double a, double b) {
+    double s = a + b;
+    DD r = {s, b - (s - a)};
+    return r;
+}
+static DD dd_add(DD a, DD b) {
+    DD s = dd_two_sum(a.hi, b.hi), t = dd_two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = dd_fast(s.hi, s.lo);
+    s.lo += t.lo;
+    return dd_fast(s.hi, s.lo);
+}
+static DD dd_neg(DD a) { DD r = {-a.hi, -a.lo}; return r; }
+static DD dd_mul(DD a, DD b) {
+    double p = a.hi * b.hi, e = fma(a.hi, b.hi, -p);
+    e += a.hi * b.lo + a.lo * b.hi;
+    return dd_fast(p, e);
+}
+static DD dd_mul_d(DD a, double b) {
+    double p = a.hi * b, e = fma(a.hi, b, -p);
+    e += a.lo * b;
+    return dd_fast(p, e);
+}
+static DD dd_div(DD a, DD b) {
+    double q1 = a.hi / b.hi;
+    DD r = dd_add(a, dd_neg(dd_mul_d(b, q1)));
+    double q2 = r.hi / b.hi;
+    r = dd_add(r, dd_neg(dd_mul_d(b, q2)));
+    double q3 = r.hi / b.hi;
+    DD q3d = {q3, 0.0};
+    return dd_add(dd_fast(q1, q2), q3d);
+}
+static const double ATAN_K16[17][2] = {  /* atan(k/16) = hi + lo */
+    {0.0, 0.0},
+    {0.06241880999595735, -1.5490756308295046e-18}, {0.12435499454676144, -3.1253241424539383e-18},
+    {0.18534794999569476, 4.180692268843079e-18}, {0.24497866312686414, 1.0698755618734451e-17},
+    {0.3028848683749714, -1.1010827903001369e-17}, {0.35877067027057225, -2.4623815582638635e-17},
+    {0.4124104415973873, -1.587652227770689e-17}, {0.4636476090008061, 2.2698777452961687e-17},
+    {0.5123894603107377, -2.5462781472855804e-17}, {0.5585993153435624, -5.4556305485916264e-18},
+    {0.6022873461349642, 2.950430737228402e-17}, {0.6435011087932844, 1.5834785051444286e-17},
+    {0.6823165548747481, 6.943223671560008e-18}, {0.7188299996216245, -2.1478388444456983e-17},
+    {0.7531512809621944, -2.4256934659182068e-17}, {0.7853981633974483, 3.061616997868383e-17}};
+static double atan2_cr(double y, double x) {
+    const DD pi = {3.141592653589793, 1.2246467991473532e-16};
+    const DD pi2 = {1.5707963267948966, 6.123233995736766e-17};
+    if (x != x || y != y) return x + y;
+    double ax = fabs(x), ay = fabs(y), res;
+    if (ay == 0.0) {
+        res = signbit(x) ? pi.hi : 0.0;
+    } else if (ax == 0.0) {
+        res = pi2.hi;
+    } else if (isinf(ay) || isinf(ax)) {
+        if (isinf(ay) && isinf(ax)) res = x > 0.0 ? 0.7853981633974483 : 2.356194490192345;
+        else if (isinf(ay)) res = pi2.hi;
+        else res = x > 0.0 ? 0.0 : pi.hi;
+    } else {
+        int swap = ay > ax;
+        double num = swap ? ax : ay, den = swap ? ay : ax;
+        double rh = num / den;
+        DD r = {rh, fma(-rh, den, num) / den};
+        int k = (int)(rh * 16.0 + 0.5);
+        double c = (double)k * 0.0625;
+        DD nu = dd_fast(r.hi - c, r.lo);
+        DD one = {1.0, 0.0};
+        DD de = dd_add(one, dd_mul_d(r, c));
+        DD u = dd_div(nu, de), w = dd_mul(u, u);
+        /* (-1)^n / (2n + 1): n = 10..5 in FP64, n = 4..1 in double-double */
+        double t = 0.047619047619047616;
+        t = -0.05263157894736842 + w.hi * t;
+        t = 0.058823529411764705 + w.hi * t;
+        t = -0.06666666666666667 + w.hi * t;
+        t = 0.07692307692307693 + w.hi * t;
+        t = -0.09090909090909091 + w.hi * t;
+        const DD c4 = {0.1111111111111111, 6.1679056923619804e-18};
+        const DD c3 = {-0.14285714285714285, -7.93016446160826e-18};
+        const DD c2 = {0.2, -1.1102230246251566e-17};
+        const DD c1 = {-0.3333333333333333, -1.850371707708594e-17};
+        DD s = dd_add(c4, dd_mul_d(w, t));
+        s = dd_add(c3, dd_mul(w, s));
+        s = dd_add(c2, dd_mul(w, s));
+        s = dd_add(c1, dd_mul(w, s));
+        DD tab = {ATAN_K16[k][0], ATAN_K16[k][1]};
+        DD a = dd_add(tab, dd_add(u, dd_mul(dd_mul(u, w), s)));
+        if (swap) a = dd_add(pi2, dd_neg(a));
+        if (x < 0.0) a = dd_add(pi, dd_neg(a));
+        res = a.hi + a.lo;
+    }
+    return signbit(y) ? -res : res;
+}
+
+/* the angle function above over arrays (test access) */
+void orc_atan2(const double *y, const double *x, int64_t n, double *out) {
+    for (int64_t k = 0; k < n; ++k) out[k] = atan2_cr(y[k], x[k]);
+}
+
 /* numpy's pairwise sum of n doubles at stride st (numpy/core/src/umath/
  * loops_utils.h pairwise_sum for n <= 128: plain loop below 8, else eight
  * running sums combined as ((0+1)+(2+3))+((4+5)+(6+7)) then the tail). */
@@ -1177,12 +1288,12 @@ static int tet_pair(const double *T1, const double *e1, const double *X1, const 
     const int np = intersect_halfplanes(hp, nh, pts);
     if (np < 0) return TET_OVERFLOW | TET_CUTS;
     if (np < 3) return TET_CUTS;
-    /* order_points (:300-318): by atan2 around the mean point, stable */
+    /* order_points (:300-318): by the angle (atan2_cr) around the mean point, stable */
     const double cx = np_sum(&pts[0][0], np, 2) / np, cy = np_sum(&pts[0][1], np, 2) / np;
     double ang[24];
     int ord[24];
     for (int k = 0; k < np; ++k) {
-        ang[k] = atan2(pts[k][1] - cy, pts[k][0] - cx);
+        ang[k] = atan2_cr(pts[k][1] - cy, pts[k][0] - cx);
         ord[k] = k;
     }
     for (int k = 1; k < np; ++k) {
@@ -1544,4 +1655,159 @@ int64_t orc_overlap_sorted(const double *a, int64_t n, const double *b, const in
         }
     }
     return cnt;
+}
+
+/* ------------------------------------------------ hydroelastic contact forces */
+
+/* LU factorisation with partial pivoting of a row-major 4x4 matrix in the
+ * order of LAPACK's unblocked getf2 (the solver np.linalg.solve reaches
+ * through LAPACK gesv): per column the first row of largest |a|, a row swap,
+ * multipliers by the pivot's reciprocal (by division below DBL_MIN), then the
+ * rank-1 update of the trailing block.  Returns 0 when a pivot is zero. */
+static int lu4(double A[16], int piv[4]) {
+    int ok = 1;
+    for (int j = 0; j < 4; ++j) {
+        int p = j;
+        double best = fabs(A[4 * j + j]);
+        for (int i = j + 1; i < 4; ++i)
+            if (fabs(A[4 * i + j]) > best) { best = fabs(A[4 * i + j]); p = i; }
+        piv[j] = p;
+        if (A[4 * p + j] != 0.0) {
+            if (p != j)
+                for (int c = 0; c < 4; ++c) { double t = A[4 * j + c]; A[4 * j + c] = A[4 * p + c]; A[4 * p + c] = t; }
+            const double d = A[4 * j + j];
+            if (fabs(d) >= DBL_MIN) {
+                const double r = 1.0 / d;
+                for (int i = j + 1; i < 4; ++i) A[4 * i + j] = A[4 * i + j] * r;
+            } else {
+                for (int i = j + 1; i < 4; ++i) A[4 * i + j] = A[4 * i + j] / d;
+            }
+        } else {
+            ok = 0;
+        }
+        for (int i = j + 1; i < 4; ++i)
+            for (int c = j + 1; c < 4; ++c) A[4 * i + c] = A[4 * i + c] - A[4 * i + j] * A[4 * j + c];
+    }
+    return ok;
+}
+
+/* getrs for one right-hand side: the row swaps in order, then the unit
+ * lower and the upper triangular solves (column-oriented, a zero entry skips
+ * its column, as the reference triangular solve does). */
+static void lu4_solve(const double A[16], const int piv[4], double b[4]) {
+    for (int j = 0; j < 4; ++j)
+        if (piv[j] != j) { double t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
+    for (int k = 0; k < 4; ++k)
+        if (b[k] != 0.0)
+            for (int i = k + 1; i < 4; ++i) b[i] = b[i] - b[k] * A[4 * i + k];
+    for (int k = 3; k >= 0; --k)
+        if (b[k] != 0.0) {
+            b[k] = b[k] / A[4 * k + k];
+            for (int i = 0; i < k; ++i) b[i] = b[i] - b[k] * A[4 * i + k];
+        }
+}
+
+/* [T^T; 1 1 1 1] of a tetrahedron (4 points, row-major (4,3)): the matrix
+ * whose inverse maps (x, 1) to barycentric coordinates
+ * (_barycentric_transform.py:4-9, _forces.py:94). */
+static void tet_matrix(const double *T, double A[16]) {
+    for (int c = 0; c < 4; ++c) {
+        A[c] = T[3 * c];
+        A[4 + c] = T[3 * c + 1];
+        A[8 + c] = T[3 * c + 2];
+        A[12 + c] = 1.0;
+    }
+}
+
+/* compute_contact_force (_forces.py:52-111): the polygon (nv points) as the
+ * fan of triangles (0, q, q + 1), at most the reference's 6 (TRIANGLES is
+ * the fan of an 8-gon, _forces.py:47-48); per triangle the pressure at its
+ * centroid from the barycentric coordinates (np.linalg.solve) weighted by the
+ * potentials times Young's modulus, times its area.  out: centre (3), force
+ * (3), area. */
+static void tet_force(const double *T, const double *eps, double E, const double *plane,
+                      const double *poly, int nv, double *out) {
+    double A[16];
+    int piv[4];
+    tet_matrix(T, A);
+    lu4(A, piv);
+    double we[4];
+    for (int r = 0; r < 4; ++r) we[r] = eps[r] * E;
+    double tf = 0.0, ta = 0.0, com[3] = {0.0, 0.0, 0.0};
+    int ntri = nv - 2;
+    if (ntri > 6) ntri = 6;
+    for (int q = 1; q <= ntri; ++q) {
+        const double *v0 = poly, *v1 = poly + 3 * q, *v2 = poly + 3 * (q + 1);
+        double b[4];
+        for (int a = 0; a < 3; ++a) b[a] = ((v0[a] + v1[a]) + v2[a]) / 3.0;
+        b[3] = 1.0;
+        const double c3[3] = {b[0], b[1], b[2]};
+        lu4_solve(A, piv, b);
+        double pr = 0.0;
+        for (int r = 0; r < 4; ++r) pr += b[r] * we[r];
+        const V e1 = mk(v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]);
+        const V e2 = mk(v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]);
+        const V cr = cross(e1, e2);
+        const double area = 0.5 * sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+        tf += pr * area;
+        ta += area;
+        for (int a = 0; a < 3; ++a) com[a] += area * c3[a];
+    }
+    if (ta > 0.0) {
+        for (int a = 0; a < 3; ++a) out[a] = com[a] / ta;
+    } else {
+        for (int a = 0; a < 3; ++a) out[a] = poly[a];
+    }
+    for (int a = 0; a < 3; ++a) out[3 + a] = tf * plane[a];
+    out[6] = ta;
+}
+
+/* Contact forces of the pairs whose status has bit 0 (rows of the other pairs
+ * are zero): T1/e1 the first body's tetrahedra and potentials, planes
+ * (m,4), polys (m,TET_MAX_POLY,3), nv (m,) as orc_tet_pairs writes them. */
+void orc_tet_contact_forces(const double *T1, const double *e1, const int32_t *pairs,
+                            const int32_t *status, const double *planes, const int32_t *nv,
+                            const double *polys, int64_t m, double E1, double *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < m; ++k) {
+        double *o = out + 7 * k;
+        if (!(status[k] & 1)) {
+            for (int c = 0; c < 7; ++c) o[c] = 0.0;
+            continue;
+        }
+        const int64_t i = pairs[2 * k];
+        tet_force(T1 + 12 * i, e1 + 4 * i, E1, planes + 4 * k, polys + 3 * TET_MAX_POLY * k, nv[k], o);
+    }
+}
+
+/* Per tetrahedron (T (n,4,3)): its AABB (n,3,2) [axis][min, max]
+ * (_mesh_processing.py:4-20) and its barycentric transform (n,4,4), the
+ * inverse of [T^T; 1] by the LU above (the reference takes the
+ * pseudo-inverse, equal to the inverse for a non-degenerate tetrahedron);
+ * flags[k] = 1 when a pivot is zero (degenerate tetrahedron). */
+void orc_tet_mesh_prep(const double *T, int64_t n, double *aabbs, double *X, int32_t *flags) {
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        const double *t = T + 12 * k;
+        for (int a = 0; a < 3; ++a) {
+            double mn = t[a], mx = t[a];
+            for (int r = 1; r < 4; ++r) {
+                const double v = t[3 * r + a];
+                if (v < mn || v != v) mn = v;
+                if (v > mx || v != v) mx = v;
+            }
+            aabbs[6 * k + 2 * a] = mn;
+            aabbs[6 * k + 2 * a + 1] = mx;
+        }
+        double A[16];
+        int piv[4];
+        tet_matrix(t, A);
+        flags[k] = lu4(A, piv) ? 0 : 1;
+        for (int c = 0; c < 4; ++c) {
+            double b[4] = {0.0, 0.0, 0.0, 0.0};
+            b[c] = 1.0;
+            lu4_solve(A, piv, b);
+            for (int r = 0; r < 4; ++r) X[16 * k + 4 * r + c] = b[r];
+        }
+    }
 }

@@ -162,6 +162,41 @@ def tet_pairs(T1, e1, X1, T2, e2, X2, pairs, youngs_modulus1=1.0, youngs_modulus
     return status, planes, nv, polys
 
 
+def atan2(y, x):
+    """The oracle's correctly rounded atan2 (the polygon ordering's angle)."""
+    y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.empty(len(y))
+    lib().orc_atan2(_p(y), _p(x), ctypes.c_int64(len(y)), _p(out))
+    return out
+
+
+def tet_contact_forces(T1, e1, pairs, status, planes, nv, polys, youngs_modulus1=1.0):
+    """compute_contact_force (_forces.py:52-111) for every pair with status
+    bit 0, from the first body's tetrahedra: (m, 7) rows of centre (3),
+    force (3), area; zero rows for the other pairs."""
+    c = lambda a, s, t=np.float64: np.ascontiguousarray(a, dtype=t).reshape(s)
+    T1 = c(T1, (-1, 4, 3)); e1 = c(e1, (-1, 4))
+    pr = c(pairs, (-1, 2), np.int32)
+    m = len(pr)
+    st = c(status, (m,), np.int32); pl = c(planes, (m, 4)); n = c(nv, (m,), np.int32)
+    po = c(polys, (m, TET_MAX_POLY, 3))
+    out = np.empty((m, 7))
+    lib().orc_tet_contact_forces(_p(T1), _p(e1), _p(pr), _p(st), _p(pl), _p(n), _p(po),
+                                 ctypes.c_int64(m), ctypes.c_double(youngs_modulus1), _p(out))
+    return out
+
+
+def tet_mesh_prep(T):
+    """Per tetrahedron: AABB (n, 3, 2) (_mesh_processing.py:4-20), barycentric
+    transform (n, 4, 4) by a partial-pivot LU inverse, degenerate flag (n,)."""
+    T = np.ascontiguousarray(T, dtype=np.float64).reshape(-1, 4, 3)
+    n = len(T)
+    aabbs = np.empty((n, 3, 2)); X = np.empty((n, 4, 4)); flags = np.empty(n, dtype=np.int32)
+    lib().orc_tet_mesh_prep(_p(T), ctypes.c_int64(n), _p(aabbs), _p(X), _p(flags))
+    return aabbs, X, flags
+
+
 def intersect_halfplanes(halfplanes):
     """_halfplanes.py:29-64; raises AssertionError where the reference does."""
     hp = np.ascontiguousarray(halfplanes, dtype=np.float64).reshape(-1, 4)

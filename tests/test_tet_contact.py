@@ -140,6 +140,34 @@ def test_oracle_plane_sign_matters():
     assert np.any(np.abs(P @ planes[k, :3] + planes[k, 3]) > 1e-6)
 
 
+def _angle_inputs(rs, n):
+    y = rs.randn(n) * np.exp(rs.uniform(-20, 20, n))
+    x = rs.randn(n) * np.exp(rs.uniform(-20, 20, n))
+    h = n // 4
+    y[:h] = x[:h] * (1 + rs.randn(h) * 1e-12)  # near the diagonals
+    y[h:2 * h] = rs.uniform(-1, 1, h)
+    x[h:2 * h] = rs.uniform(-1, 1, h)
+    sp = [0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0, 1e-310, -1e-310, 1e300]
+    yy, xx = np.meshgrid(sp, sp)
+    return np.concatenate([y, yy.ravel()]), np.concatenate([x, xx.ravel()])
+
+
+def test_oracle_angle_is_correctly_rounded():
+    """The ordering's atan2 against 50-digit mpmath on a sample (including
+    near-diagonal third-quadrant inputs, where np.arctan2 and the C library
+    misround on some hosts) and against np.arctan2's special cases."""
+    mpmath = pytest.importorskip("mpmath")
+    rs = np.random.RandomState(12)
+    y, x = _angle_inputs(rs, 4000)
+    got = oracle.atan2(y, x)
+    sp = slice(4000, None)
+    assert np.array_equal(got[sp], np.arctan2(y[sp], x[sp]), equal_nan=True)
+    assert np.array_equal(np.signbit(got[sp]), np.signbit(np.arctan2(y[sp], x[sp])))
+    mpmath.mp.dps = 50
+    for k in range(4000):
+        assert got[k] == float(mpmath.atan2(mpmath.mpf(y[k]), mpmath.mpf(x[k]))), (y[k], x[k])
+
+
 @pytest.mark.gpu
 def test_tet_pairs_match_oracle(gpu):
     """Random pairs over two bodies (overlapping and not, shared
@@ -201,3 +229,15 @@ def test_tet_dropin(gpu):
     hit, (plane, poly) = hc.intersect_tetrahedron_pair(T1[i], e1[i], X1[i], T2[j], e2[j], X2[j])
     assert not hit and np.array_equal(plane, wp[miss])
     assert (poly is None) == (not st[miss] & 0x20)
+
+
+@pytest.mark.gpu
+def test_polygon_angles_match_oracle(gpu):
+    """The kernels' atan2 equals the oracle's bit for bit."""
+    from distance3d_amd import kernels
+    rs = np.random.RandomState(13)
+    y, x = _angle_inputs(rs, 1 << 20)
+    got = kernels.polygon_angles(y, x).cpu().numpy()
+    want = oracle.atan2(y, x)
+    assert np.array_equal(got, want, equal_nan=True)
+    assert np.array_equal(np.signbit(got), np.signbit(want))
