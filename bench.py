@@ -55,7 +55,7 @@ def parse():
     p.add_argument("--no-parity", action="store_true", help="skip the oracle parity checks")
     p.add_argument("--no-extra", action="store_true", help="only the headline workload")
     p.add_argument("--only", type=str, default=None,
-                   help="comma list of extra configs: hull,epa,broad,serving,mpr,tet")
+                   help="comma list of extra configs: hull,epa,broad,serving,mpr,tet,contact")
     p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--serving-requests", type=int, default=50000)
     p.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -455,6 +455,72 @@ def run_tet(args, steps=5, warmup=1):
     return res
 
 
+def run_contact(args, steps=10, warmup=2):
+    """Hydroelastic contact surface of two overlapping icosphere bodies
+    (order 4: 5,120 tetrahedra each, centres 0.14 apart at radius 0.15):
+    tetrahedron boxes and barycentric transforms, brute-force overlap of the
+    boxes (26 M box tests, 5.1 M candidate pairs, row-major as the
+    reference lists them), status pass, compaction of the
+    intersecting pairs, polygon and force pass.  Meshes resident in HBM; the
+    step includes its two host reads (pair count, hit count)."""
+    import torch
+    from distance3d_amd import hydroelastic_contact as hc, kernels
+    rb1 = hc.RigidBody.make_sphere(np.array([0.0, 0.0, 0.01]), 0.15, 4)
+    rb2 = hc.RigidBody.make_sphere(np.array([0.0, 0.0, 0.15]), 0.15, 4)
+    rb1.express_in(rb2.body2origin_)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    T1, T2 = d(rb1.tetrahedra_points), d(rb2.tetrahedra_points)
+    e1, e2 = d(rb1.tetrahedra_potentials), d(rb2.tetrahedra_potentials)
+    counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+    out = {}
+
+    def step():
+        ab1, X1, _ = kernels.tet_mesh_prep(T1)
+        ab2, X2, _ = kernels.tet_mesh_prep(T2)
+        pairs = kernels.overlap_dense(ab1, ab2, cap=6000000, counter=counter)
+        st = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, 1.0, 1.0, geometry=False)[0]
+        hp = pairs[torch.nonzero(st & 1).squeeze(1)]
+        st2, planes, nv, polys = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, hp, 1.0, 1.0)
+        F = kernels.tet_contact_forces(T1, e1, hp, st2, planes, nv, polys, 1.0)
+        out.update(pairs=pairs, hp=hp, planes=planes, nv=nv, polys=polys, F=F)
+
+    wall, ev = timed_steps(step, steps, warmup)
+    n_pairs, n_hits = int(out["pairs"].shape[0]), int(out["hp"].shape[0])
+    res = {"workload": "hydroelastic contact surface, two order-4 icosphere bodies (5,120 "
+                       "tetrahedra each, deep overlap): prep, broad phase, pair contact, forces",
+           "tetrahedra": [int(T1.shape[0]), int(T2.shape[0])], "candidate_pairs": n_pairs,
+           "intersecting_pairs": n_hits, "ms_per_step": round(wall, 4),
+           "value": round(n_pairs / (wall / 1e3), 1), "unit": "candidate tetrahedron pairs/s"}
+    if not args.no_parity:
+        from oracle import oracle
+        t0 = time.perf_counter()
+        P1, P2 = rb1.tetrahedra_points, rb2.tetrahedra_points
+        a1, Y1, _ = oracle.tet_mesh_prep(P1)
+        a2, Y2, _ = oracle.tet_mesh_prep(P2)
+        cap = n_pairs + 1
+        wp = np.empty((cap, 2), dtype=np.int64)
+        k = oracle.lib().orc_all_aabbs_overlap(oracle._p(a1), len(a1), oracle._p(a2), len(a2),
+                                               oracle._p(wp), cap)
+        wp = wp[:k]
+        st, pl, nvw, po = oracle.tet_pairs(P1, rb1.tetrahedra_potentials, Y1, P2,
+                                           rb2.tetrahedra_potentials, Y2, wp)
+        Fw = oracle.tet_contact_forces(P1, rb1.tetrahedra_potentials, wp, st, pl, nvw, po)
+        cpu_s = time.perf_counter() - t0
+        hit = np.nonzero(st & 1)[0]
+        same = (k == n_pairs and np.array_equal(out["pairs"].cpu().numpy(), wp)
+                and np.array_equal(out["hp"].cpu().numpy(), wp[hit])
+                and np.array_equal(out["planes"].cpu().numpy(), pl[hit])
+                and np.array_equal(out["nv"].cpu().numpy(), nvw[hit])
+                and np.array_equal(out["polys"].cpu().numpy(), po[hit], equal_nan=True)
+                and np.array_equal(out["F"].cpu().numpy(), Fw[hit]))
+        res["parity"] = {"pairs_hits_planes_polygons_forces_bit_identical": bool(same)}
+        res["cpu_oracle"] = {"seconds": round(cpu_s, 3), "threads": oracle.num_threads(),
+                             "note": "brute-force box overlaps on one core, pair contact and "
+                                     "forces on all host threads"}
+    log("contact: %.3f ms/step, %d candidate pairs, %d hits" % (wall, n_pairs, n_hits))
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -518,7 +584,7 @@ def main():
         log("one pair per launch: %.3g pairs/s" % opl["value"])
     if not args.no_extra:
         only = (set(args.only.split(",")) if args.only
-                else {"hull", "epa", "broad", "serving", "mpr", "tet"})
+                else {"hull", "epa", "broad", "serving", "mpr", "tet", "contact"})
         cfgs = {}
         if "hull" in only:
             cfgs["hull_pairs"] = run_pipeline_config(workloads.hull_pairs(), args, lanes, 20, 3,
@@ -539,6 +605,8 @@ def main():
             cfgs["mpr"] = run_mpr(w, args)
         if "tet" in only:
             cfgs["tet_pairs"] = run_tet(args)
+        if "contact" in only:
+            cfgs["contact_surface"] = run_contact(args)
         result["configs"] = cfgs
     print(json.dumps(result), flush=True)
 

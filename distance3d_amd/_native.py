@@ -84,6 +84,7 @@ _SIGNATURES = {
     "d3d_bvh_build": [_P, _I64, _P, _I64, _P],
     "d3d_bvh_query": [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P],
     "d3d_bvh_refit": [_P, _I64, _P, _I64, _P],
+    "d3d_overlap_dense": [_P, _I64, _P, _I64, _P, _I64, _P, _P, _I64, _P],
     "d3d_mpr_batch": [_P, _P, _P, _P, _I64, _D, _I32, _I32, _P, _P, _P],
     "d3d_tet_pairs_batch": [_P, _P, _P, _P, _P, _P, _P, _I64, _D, _D, _P, _P, _P, _P, _P, _I64,
                             _P],
@@ -103,7 +104,8 @@ _initialised = False
 def exported_symbols():
     """Names of the C-ABI functions this binding uses."""
     return (["d3d_abi_version", "d3d_last_error", "d3d_init", "d3d_gjk_workspace_bytes",
-             "d3d_bvh_bytes", "d3d_gjk_epa_workspace_bytes", "d3d_tet_pairs_workspace_bytes"]
+             "d3d_bvh_bytes", "d3d_gjk_epa_workspace_bytes", "d3d_tet_pairs_workspace_bytes",
+             "d3d_overlap_dense_workspace_bytes"]
             + list(_SIGNATURES))
 
 
@@ -125,7 +127,8 @@ def load_library(path=LIB_PATH):
             lib.d3d_init.argtypes = [ctypes.c_int]
             lib.d3d_init.restype = ctypes.c_int
             for name in ("d3d_gjk_workspace_bytes", "d3d_bvh_bytes",
-                         "d3d_gjk_epa_workspace_bytes", "d3d_tet_pairs_workspace_bytes"):
+                         "d3d_gjk_epa_workspace_bytes", "d3d_tet_pairs_workspace_bytes",
+                         "d3d_overlap_dense_workspace_bytes"):
                 getattr(lib, name).argtypes = [ctypes.c_int64]
                 getattr(lib, name).restype = ctypes.c_int64
             for name, argtypes in _SIGNATURES.items():

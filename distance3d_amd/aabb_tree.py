@@ -114,7 +114,10 @@ def all_aabbs_overlap(aabbs1, aabbs2):
     if len(a1) == 0 or len(a2) == 0:
         e = np.empty(0, dtype=np.int64)
         return e, e, []
-    pairs = kernels.bvh_query(kernels.bvh_build(a2), a1).cpu().numpy().astype(np.int64)
+    if len(a1) * len(a2) <= kernels.DENSE_OVERLAP_MAX_TESTS:
+        pairs = kernels.overlap_dense(a1, a2).cpu().numpy().astype(np.int64)
+    else:
+        pairs = kernels.bvh_query(kernels.bvh_build(a2), a1).cpu().numpy().astype(np.int64)
     return np.unique(pairs[:, 0]), np.unique(pairs[:, 1]), list(map(tuple, pairs.tolist()))
 
 

@@ -482,6 +482,98 @@ __global__ __launch_bounds__(kBvhBlock) void bvh_query_kernel(
 
 __global__ void bvh_zero_counter(unsigned long long *c) { *c = 0ull; }
 
+// ------------------------------------------------------------ dense overlap
+//
+// all_aabbs_overlap (aabb_tree.py:465-500) by brute force, for box sets
+// whose product n * m is small enough that testing every pair beats a tree
+// walk (dense overlaps: a tree query with ~1000 hits is a long serial walk).
+// One wavefront takes kDenseRows rows of `a` (the boxes stay in scalar
+// registers) and sweeps `b` 64 boxes at a time, one per lane; a count pass,
+// an exclusive scan, and a write pass that places each row's hits at the
+// row's offset in b order (ballot ranks), so the pairs come out row-major
+// (i outer, j inner), the reference's order, with no sort.
+constexpr int kDenseRows = 4;
+constexpr int kDenseBlock = 256;
+
+__device__ __forceinline__ bool box_overlap(const double (&q)[6], const double *__restrict__ B) {
+    // aabb_overlap(a_i, b_j), aabb_tree.py:520-527 (closed intervals)
+    return q[0] <= B[1] && q[1] >= B[0] && q[2] <= B[3] && q[3] >= B[2] && q[4] <= B[5] &&
+           q[5] >= B[4];
+}
+
+template <bool kWrite>
+__global__ __launch_bounds__(kDenseBlock) void dense_overlap_kernel(
+    const double *__restrict__ a, int64_t n, const double *__restrict__ b, int64_t m,
+    unsigned long long *__restrict__ counts, int32_t *__restrict__ pairs, int64_t cap) {
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * kDenseBlock + threadIdx.x) / kWave;
+    const int64_t row0 = wave * kDenseRows;
+    if (row0 >= n) return;
+    double q[kDenseRows][6];
+    unsigned long long off[kDenseRows];
+#pragma unroll
+    for (int r = 0; r < kDenseRows; ++r) {
+        const int64_t row = min(row0 + r, n - 1);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) q[r][c] = wave_uniform(a[6 * row + c]);
+        off[r] = kWrite ? counts[row] : 0ull;
+    }
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+    for (int64_t j0 = 0; j0 < m; j0 += kWave) {
+        const int64_t j = j0 + lane;
+        double B[6];
+        const bool in = j < m;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) B[c] = in ? b[6 * j + c] : 0.0;
+#pragma unroll
+        for (int r = 0; r < kDenseRows; ++r) {
+            const bool hit = in && row0 + r < n && box_overlap(q[r], B);
+            const unsigned long long msk = __ballot(hit);
+            if (kWrite && hit) {
+                const unsigned long long pos = off[r] + (unsigned long long)__popcll(msk & lt);
+                if (pos < (unsigned long long)cap) {
+                    pairs[2 * pos] = (int32_t)(row0 + r);
+                    pairs[2 * pos + 1] = (int32_t)j;
+                }
+            }
+            off[r] += (unsigned long long)__popcll(msk);
+        }
+    }
+    if (!kWrite && lane < kDenseRows && row0 + lane < n) {
+        unsigned long long mine = off[0];
+#pragma unroll
+        for (int r = 1; r < kDenseRows; ++r) mine = lane == r ? off[r] : mine;
+        counts[row0 + lane] = mine;
+    }
+}
+
+// Exclusive scan of counts[0..n) in place by one block; the sum to *total.
+__global__ __launch_bounds__(1024) void dense_scan_kernel(unsigned long long *__restrict__ counts,
+                                                          int64_t n,
+                                                          unsigned long long *__restrict__ total) {
+    __shared__ unsigned long long part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t lo = min((int64_t)t * per, n), hi = min(lo + per, n);
+    unsigned long long sum = 0;
+    for (int64_t k = lo; k < hi; ++k) sum += counts[k];
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const unsigned long long v = t >= d ? part[t - d] : 0ull;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned long long run = part[t] - sum;
+    for (int64_t k = lo; k < hi; ++k) {
+        const unsigned long long c = counts[k];
+        counts[k] = run;
+        run += c;
+    }
+    if (t == 1023) *total = part[1023];
+}
+
 }  // namespace d3d
 
 using namespace d3d;
@@ -583,5 +675,40 @@ extern "C" int d3d_bvh_query(const void *bvh, int64_t n, const double *aabbs, co
     hipLaunchKernelGGL(bvh_query_kernel, dim3(nb), dim3(kBvhBlock), 0, st, v.nodes, v.leaf_orig, n,
                        aabbs, queries, m, pairs, cap, (unsigned long long *)n_pairs);
     D3D_CHECK_LAUNCH("bvh_query_kernel");
+    return D3D_OK;
+}
+
+extern "C" int64_t d3d_overlap_dense_workspace_bytes(int64_t n) {
+    if (n < 0) return -1;
+    return (n + 1) * (int64_t)sizeof(unsigned long long);
+}
+
+extern "C" int d3d_overlap_dense(const double *a, int64_t n, const double *b, int64_t m,
+                                 int32_t *pairs, int64_t cap, uint64_t *n_pairs, void *workspace,
+                                 int64_t workspace_bytes, void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(n >= 0 && n < (int64_t)0x7fffffff && m >= 0 && m < (int64_t)0x7fffffff,
+                  "n and m must fit in int32");
+    D3D_CHECK_ARG(n_pairs && (n == 0 || (a && workspace)) && (m == 0 || b), "NULL buffer");
+    D3D_CHECK_ARG(cap >= 0 && (cap == 0 || pairs), "pairs buffer is NULL");
+    D3D_CHECK_ARG(workspace_bytes >= d3d_overlap_dense_workspace_bytes(n), "workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(bvh_zero_counter, dim3(1), dim3(1), 0, st, (unsigned long long *)n_pairs);
+    D3D_CHECK_LAUNCH("bvh_zero_counter");
+    if (n == 0 || m == 0) return D3D_OK;
+    unsigned long long *counts = (unsigned long long *)workspace;
+    const int64_t waves = (n + kDenseRows - 1) / kDenseRows;
+    const unsigned blocks = (unsigned)((waves * kWave + kDenseBlock - 1) / kDenseBlock);
+    hipLaunchKernelGGL(dense_overlap_kernel<false>, dim3(blocks), dim3(kDenseBlock), 0, st, a, n, b,
+                       m, counts, pairs, cap);
+    D3D_CHECK_LAUNCH("dense_overlap_kernel<count>");
+    hipLaunchKernelGGL(dense_scan_kernel, dim3(1), dim3(1024), 0, st, counts, n,
+                       (unsigned long long *)n_pairs);
+    D3D_CHECK_LAUNCH("dense_scan_kernel");
+    if (cap > 0) {
+        hipLaunchKernelGGL(dense_overlap_kernel<true>, dim3(blocks), dim3(kDenseBlock), 0, st, a, n,
+                           b, m, counts, pairs, cap);
+        D3D_CHECK_LAUNCH("dense_overlap_kernel<write>");
+    }
     return D3D_OK;
 }

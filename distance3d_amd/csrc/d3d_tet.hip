@@ -62,15 +62,18 @@ __global__ __launch_bounds__(kTetBlock) void tet_pair_kernel(
     __shared__ double ang_s[kTetMaxPoly][kTetBlock];
     __shared__ unsigned char ord_s[kTetMaxPoly][kTetBlock];
     const int lane = threadIdx.x;
-    const int64_t k = (int64_t)blockIdx.x * kTetBlock + lane;
-    if (k >= m) return;
+    // grid-stride over the pairs; the 2-D polygon points of the lane's
+    // current pair live in its own workspace row
+    double *pts = work + 2 * kTetMaxPoly * ((int64_t)blockIdx.x * kTetBlock + lane);
+    for (int64_t k = (int64_t)blockIdx.x * kTetBlock + lane; k < m;
+         k += (int64_t)gridDim.x * kTetBlock) {
     const int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
     const double *t1 = T1 + 12 * i, *t2 = T2 + 12 * j;
     const double *x1 = X1 + 16 * i, *x2 = X2 + 16 * j;
     const double *p1 = e1 + 4 * i, *p2 = e2 + 4 * j;
-    double *plane = planes + 4 * k;
-    double *poly = polys + 3 * kTetMaxPoly * k;
-    double *pts = work + 2 * kTetMaxPoly * k;
+    // planes and polygons are optional outputs (NULL: status and vertex
+    // count only)
+    double *poly = polys ? polys + 3 * kTetMaxPoly * k : nullptr;
     int status = 0, nv = 0;
 
     // contact_plane (_tetrahedron_intersection.py:176-229)
@@ -108,9 +111,10 @@ __global__ __launch_bounds__(kTetBlock) void tet_pair_kernel(
         const double d = sqrt(dot(pp, pp));
         const v3 n = d > 0.0 ? mk(pp.x / d, pp.y / d, pp.z / d) : mk(0.0, 0.0, 1.0);
         h[0] = n.x; h[1] = n.y; h[2] = n.z; h[3] = d;
-        for (int q = 0; q < 3; ++q) {
-            poly[3 * q] = pp.x; poly[3 * q + 1] = pp.y; poly[3 * q + 2] = pp.z;
-        }
+        if (poly)
+            for (int q = 0; q < 3; ++q) {
+                poly[3 * q] = pp.x; poly[3 * q + 1] = pp.y; poly[3 * q + 2] = pp.z;
+            }
         nv = 3;
         status = 1 | kTetSame;
     } else {
@@ -223,9 +227,11 @@ __global__ __launch_bounds__(kTetBlock) void tet_pair_kernel(
                         const double dx = ux - pts[2 * pr], dy = uy - pts[2 * pr + 1];
                         if (!(sqrt(dx * dx + dy * dy) > 10.0 * kEps)) continue;
                     }
-                    poly[3 * nu] = (ux * ax.x + uy * ay.x) + pp.x;
-                    poly[3 * nu + 1] = (ux * ax.y + uy * ay.y) + pp.y;
-                    poly[3 * nu + 2] = (ux * ax.z + uy * ay.z) + pp.z;
+                    if (poly) {
+                        poly[3 * nu] = (ux * ax.x + uy * ay.x) + pp.x;
+                        poly[3 * nu + 1] = (ux * ax.y + uy * ay.y) + pp.y;
+                        poly[3 * nu + 2] = (ux * ax.z + uy * ay.z) + pp.z;
+                    }
                     ++nu;
                 }
                 if (nu >= 3) {
@@ -235,10 +241,15 @@ __global__ __launch_bounds__(kTetBlock) void tet_pair_kernel(
             }
         }
     }
-    plane[0] = h[0]; plane[1] = h[1]; plane[2] = h[2]; plane[3] = h[3];
-    for (int q = 3 * nv; q < 3 * kTetMaxPoly; ++q) poly[q] = NAN;
+    if (planes) {
+        double *plane = planes + 4 * k;
+        plane[0] = h[0]; plane[1] = h[1]; plane[2] = h[2]; plane[3] = h[3];
+    }
+    if (poly)
+        for (int q = 3 * nv; q < 3 * kTetMaxPoly; ++q) poly[q] = NAN;
     status_out[k] = status;
     nv_out[k] = nv;
+    }
 }
 
 // LU factorisation with partial pivoting of a 4x4 matrix in the order of
@@ -445,9 +456,14 @@ __global__ __launch_bounds__(kTetBlock) void angle_kernel(const double *__restri
 
 using namespace d3d;
 
+// blocks of the grid-stride pair kernel (4096 x 64 lanes: more than the
+// chip holds at the kernel's occupancy); the workspace has one row per lane
+constexpr int64_t kTetMaxBlocks = 4096;
+
 extern "C" int64_t d3d_tet_pairs_workspace_bytes(int64_t m) {
     if (m < 0) return -1;
-    return m * 2 * kTetMaxPoly * (int64_t)sizeof(double);
+    const int64_t lanes = min((int64_t)grid_for(m, kTetBlock), kTetMaxBlocks) * kTetBlock;
+    return lanes * 2 * kTetMaxPoly * (int64_t)sizeof(double);
 }
 
 extern "C" int d3d_tet_pairs_batch(const double *T1, const double *e1, const double *X1,
@@ -459,11 +475,11 @@ extern "C" int d3d_tet_pairs_batch(const double *T1, const double *e1, const dou
     clear_error();
     D3D_CHECK_ARG(m >= 0, "m must be >= 0");
     if (m == 0) return D3D_OK;
-    D3D_CHECK_ARG(T1 && e1 && X1 && T2 && e2 && X2 && pairs && status && planes && n_vertices &&
-                      polygons && workspace,
+    D3D_CHECK_ARG(T1 && e1 && X1 && T2 && e2 && X2 && pairs && status && n_vertices && workspace,
                   "NULL buffer");
     D3D_CHECK_ARG(workspace_bytes >= d3d_tet_pairs_workspace_bytes(m), "workspace too small");
-    hipLaunchKernelGGL(tet_pair_kernel, dim3(grid_for(m, kTetBlock)), dim3(kTetBlock), 0,
+    const unsigned blocks = (unsigned)min((int64_t)grid_for(m, kTetBlock), kTetMaxBlocks);
+    hipLaunchKernelGGL(tet_pair_kernel, dim3(blocks), dim3(kTetBlock), 0,
                        (hipStream_t)stream, T1, e1, X1, T2, e2, X2, pairs, m, youngs_modulus1,
                        youngs_modulus2, status, planes, n_vertices, polygons, (double *)workspace);
     D3D_CHECK_LAUNCH("tet_pair_kernel");

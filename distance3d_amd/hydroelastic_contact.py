@@ -65,6 +65,27 @@ def _check(status):
         raise AssertionError("more half-plane intersection points than the reference reserves")
 
 
+def _intersecting(T1, e1, X1, T2, e2, X2, pairs, E1, E2, forces):
+    """The intersecting pairs of a pair list, in list order, on the GPU: a
+    status-only pass over all pairs, the hit rows compacted on the device,
+    then planes, polygons (and forces) for the hits only.  Returns host
+    arrays (hit pairs (k,2), planes (k,4), n_vertices (k,), polygons
+    (k,24,3), forces (k,7) or None)."""
+    import torch
+    pr = torch.as_tensor(pairs)
+    pr = (pr if pr.is_cuda else pr.cuda()).to(torch.int32).reshape(-1, 2).contiguous()
+    st, _, _, _ = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, pr, E1, E2, geometry=False)
+    _check(np.array([int((st & OVERFLOW).max())]) if len(st) else np.zeros(1, dtype=np.int32))
+    hp = pr[torch.nonzero(st & 1).squeeze(1)]
+    if hp.shape[0] == 0:
+        return (np.empty((0, 2), dtype=np.int64), np.empty((0, 4)), np.empty(0, dtype=np.int32),
+                np.empty((0, kernels.TET_MAX_POLY, 3)), np.empty((0, 7)) if forces else None)
+    st2, planes, nv, polys = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, hp, E1, E2)
+    F = kernels.tet_contact_forces(T1, e1, hp, st2, planes, nv, polys, E1) if forces else None
+    return (hp.cpu().numpy().astype(np.int64), planes.cpu().numpy(), nv.cpu().numpy(),
+            polys.cpu().numpy(), F.cpu().numpy() if forces else None)
+
+
 def intersect_tetrahedron_pairs(pairs, tetrahedra_points1, tetrahedra_points2, epsilon1, epsilon2,
                                 X1, X2, youngs_modulus1=1.0, youngs_modulus2=1.0):
     """Intersect pairs of tetrahedra (_tetrahedron_intersection.py:7-85).
@@ -77,15 +98,11 @@ def intersect_tetrahedron_pairs(pairs, tetrahedra_points1, tetrahedra_points2, e
     pr = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
     if len(pr) == 0:
         return False, [], [], [], []
-    st, planes, nv, polys = kernels.tet_pairs_batch(
-        T1, epsilon1, _dense(X1, len(T1)), T2, epsilon2, _dense(X2, len(T2)), pr,
-        youngs_modulus1, youngs_modulus2)
-    st, planes, nv, polys = st.cpu().numpy(), planes.cpu().numpy(), nv.cpu().numpy(), polys.cpu().numpy()
-    _check(st)
-    hit = np.nonzero(st & 1)[0]
-    contact_planes = planes[hit] if len(hit) else []
-    return (len(hit) > 0, contact_planes, [polys[k, :nv[k]] for k in hit],
-            pr[hit, 0].tolist(), pr[hit, 1].tolist())
+    hp, planes, nv, polys, _ = _intersecting(T1, epsilon1, _dense(X1, len(T1)), T2, epsilon2,
+                                             _dense(X2, len(T2)), pr, youngs_modulus1,
+                                             youngs_modulus2, False)
+    return (len(hp) > 0, planes if len(hp) else [], [polys[k, :nv[k]] for k in range(len(hp))],
+            hp[:, 0].tolist(), hp[:, 1].tolist())
 
 
 def intersect_tetrahedron_pair(tetrahedron1, epsilon1, X1, tetrahedron2, epsilon2, X2,
@@ -305,17 +322,6 @@ class ContactSurface:
                 _transform_points(frame2world, t2.reshape(n * 4, 3)).reshape(n, 4, 3))
 
 
-def _contact_batch(T1, e1, X1, T2, e2, X2, pairs, E1, E2):
-    """Planes, polygons and forces of the pairs on the GPU, back on the host:
-    (hit rows, planes, n_vertices, polygons, forces)."""
-    st, planes, nv, polys = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, E1, E2)
-    forces = kernels.tet_contact_forces(T1, e1, pairs, st, planes, nv, polys, E1)
-    st = st.cpu().numpy()
-    _check(st)
-    hit = np.nonzero(st & 1)[0]
-    return hit, planes.cpu().numpy(), nv.cpu().numpy(), polys.cpu().numpy(), forces.cpu().numpy()
-
-
 def find_contact_surface(rigid_body1, rigid_body2, use_aabb_trees=False):
     """Contact surface of two rigid bodies (_interface.py:52-101); like the
     reference it re-expresses rigid_body1 in rigid_body2's frame.  Broad phase
@@ -330,6 +336,8 @@ def find_contact_surface(rigid_body1, rigid_body2, use_aabb_trees=False):
     if use_aabb_trees:
         _, _, _, pairs = rigid_body1.aabb_tree.overlaps_aabb_tree(rigid_body2.aabb_tree)
         pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    elif len(ab1) * len(ab2) <= kernels.DENSE_OVERLAP_MAX_TESTS:
+        pairs = kernels.overlap_dense(ab1, ab2)
     elif len(ab1) and len(ab2):
         pairs = kernels.bvh_query(kernels.bvh_build(ab2), ab1)
     else:
@@ -339,17 +347,14 @@ def find_contact_surface(rigid_body1, rigid_body2, use_aabb_trees=False):
         surface = ContactSurface(rigid_body2.body2origin_, False, np.empty((0, 4)), [], [], [])
         surface.add_polygon_info(np.empty(0), np.empty((0, 3)), np.empty((0, 3)), [])
         return surface
-    pr = torch.as_tensor(pairs).cuda().to(torch.int32)
-    hit, planes, nv, polys, forces = _contact_batch(
-        T1, rigid_body1.tetrahedra_potentials, X1, T2, rigid_body2.tetrahedra_potentials, X2, pr,
-        E1, E2)
-    pr = pr.cpu().numpy()
-    surface = ContactSurface(rigid_body2.body2origin_, len(hit) > 0,
-                             planes[hit] if len(hit) else np.empty((0, 4)),
-                             [polys[k, :nv[k]] for k in hit], pr[hit, 0].tolist(),
-                             pr[hit, 1].tolist())
-    surface.add_polygon_info(forces[hit, 6], forces[hit, 0:3], forces[hit, 3:6],
-                             [TRIANGLES[:nv[k] - 2] for k in hit])
+    hp, planes, nv, polys, F = _intersecting(
+        T1, rigid_body1.tetrahedra_potentials, X1, T2, rigid_body2.tetrahedra_potentials, X2,
+        pairs, E1, E2, True)
+    k = len(hp)
+    surface = ContactSurface(rigid_body2.body2origin_, k > 0, planes,
+                             [polys[q, :nv[q]] for q in range(k)], hp[:, 0].tolist(),
+                             hp[:, 1].tolist())
+    surface.add_polygon_info(F[:, 6], F[:, 0:3], F[:, 3:6], [TRIANGLES[:nv[q] - 2] for q in range(k)])
     return surface
 
 

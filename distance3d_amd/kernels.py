@@ -159,11 +159,11 @@ TET_MAX_POLY = 24
 
 
 def tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, youngs_modulus1=1.0, youngs_modulus2=1.0,
-                    stream=None):
+                    stream=None, geometry=True):
     """Contact plane and polygon of tetrahedron pairs (d3d_tet_pairs_batch).
     T (n,4,3), e (n,4), X (n,4,4) per body, pairs (m,2); tensors or arrays.
     Returns device tensors (status (m,), planes (m,4), n_vertices (m,),
-    polygons (m,24,3))."""
+    polygons (m,24,3)); planes and polygons are None when not ``geometry``."""
     import torch
 
     def dev(a, shape):
@@ -180,9 +180,9 @@ def tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, youngs_modulus1=1.0, youngs_m
               or int(pr.min()) < 0):
         raise ValueError("pair index out of range")
     status = torch.empty(m, dtype=torch.int32, device="cuda")
-    planes = torch.empty((m, 4), dtype=torch.float64, device="cuda")
+    planes = torch.empty((m, 4), dtype=torch.float64, device="cuda") if geometry else None
     nv = torch.empty(m, dtype=torch.int32, device="cuda")
-    polys = torch.empty((m, TET_MAX_POLY, 3), dtype=torch.float64, device="cuda")
+    polys = torch.empty((m, TET_MAX_POLY, 3), dtype=torch.float64, device="cuda") if geometry else None
     ws = torch.empty(max(int(_native.load_library().d3d_tet_pairs_workspace_bytes(m)), 1),
                      dtype=torch.uint8, device="cuda")
     check(lib().d3d_tet_pairs_batch(ptr(T1), ptr(e1), ptr(X1), ptr(T2), ptr(e2), ptr(X2), ptr(pr),
@@ -336,6 +336,35 @@ def bvh_refit(bvh, aabbs, stream=None):
     check(lib().d3d_bvh_refit(ptr(bvh.aabbs), bvh.n, ptr(bvh.buffer), bvh.buffer.numel(),
                               stream_ptr(stream)), "d3d_bvh_refit")
     return bvh
+
+
+# below this many box tests (n * m) overlap_dense beats a BVH query
+DENSE_OVERLAP_MAX_TESTS = 1 << 28
+
+
+def overlap_dense(a, b, cap=None, counter=None, stream=None):
+    """All (i, j) with closed-interval overlap of a[i] and b[j], by testing
+    every pair (d3d_overlap_dense), row-major (i, then j ascending), as an
+    (k, 2) int32 device tensor.  Reads the pair count back (one host
+    synchronisation); a second pass runs when ``cap`` was too small."""
+    import torch
+    a = _dev64(a, (-1, 3, 2))
+    b = _dev64(b, (-1, 3, 2)).to(a.device)
+    n, m = a.shape[0], b.shape[0]
+    if counter is None:
+        counter = torch.zeros(1, dtype=torch.int64, device=a.device)
+    ws = torch.empty(max(int(_native.load_library().d3d_overlap_dense_workspace_bytes(n)), 8),
+                     dtype=torch.uint8, device=a.device)
+    if cap is None:
+        cap = 4 * n + 1024
+    while True:
+        pairs = torch.empty((max(cap, 1), 2), dtype=torch.int32, device=a.device)
+        check(lib().d3d_overlap_dense(ptr(a), n, ptr(b), m, ptr(pairs), cap, ptr(counter), ptr(ws),
+                                      ws.numel(), stream_ptr(stream)), "d3d_overlap_dense")
+        k = int(counter.item())
+        if k <= cap:
+            return pairs[:k]
+        cap = k
 
 
 def bvh_query(bvh, queries, cap=None, counter=None, stream=None, sort=True):

@@ -298,7 +298,10 @@ int d3d_mpr_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *
  * intersection points than the reference reserves, where it asserts), the
  * contact plane in Hesse normal form planes (m,4), and the contact polygon
  * in counter-clockwise order n_vertices (m), polygons (m,24,3) (NaN past
- * n_vertices).  workspace: d3d_tet_pairs_workspace_bytes(m) device bytes.
+ * n_vertices).  planes and polygons may be NULL (status and n_vertices
+ * only, e.g. a first pass that finds the intersecting pairs of a large
+ * broad-phase list).  workspace: d3d_tet_pairs_workspace_bytes(m) device
+ * bytes (at most 100 MB whatever m).
  */
 int64_t d3d_tet_pairs_workspace_bytes(int64_t m);
 int d3d_tet_pairs_batch(const double *T1, const double *e1, const double *X1, const double *T2,
@@ -362,6 +365,20 @@ int64_t d3d_bvh_bytes(int64_t n);
 int d3d_bvh_build(const double *aabbs, int64_t n, void *bvh, int64_t bvh_bytes, void *stream);
 int d3d_bvh_query(const void *bvh, int64_t n, const double *aabbs, const double *queries,
                   int64_t m, int32_t *pairs, int64_t cap, uint64_t *n_pairs, void *stream);
+
+/*
+ * all_aabbs_overlap (aabb_tree.py:465-500) by brute force: every (i, j) with
+ * aabb_overlap(a[i], b[j]) -- a (n,3,2), b (m,3,2) f64 --, written to pairs
+ * (cap,2) int32 in the reference's row-major order (i ascending, then j
+ * ascending), the total count to the device uint64 *n_pairs (pairs beyond
+ * cap are counted, not written).  Tests all n*m pairs: for small or densely
+ * overlapping sets, where it is faster than a BVH query and needs no sort.
+ * workspace: d3d_overlap_dense_workspace_bytes(n) device bytes.
+ */
+int64_t d3d_overlap_dense_workspace_bytes(int64_t n);
+int d3d_overlap_dense(const double *a, int64_t n, const double *b, int64_t m, int32_t *pairs,
+                      int64_t cap, uint64_t *n_pairs, void *workspace, int64_t workspace_bytes,
+                      void *stream);
 
 /*
  * d3d_bvh_refit(aabbs (n,3,2) f64, n, bvh, bytes, stream): new boxes for the

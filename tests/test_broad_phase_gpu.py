@@ -152,6 +152,28 @@ def test_all_aabbs_overlap_row_major(gpu):  # aabb_tree.py:465-500
     assert aabb_tree.aabb_overlap(a[o[0, 0]], b[o[0, 1]])
 
 
+def test_overlap_dense_row_major(gpu):  # aabb_tree.py:465-500
+    """Brute-force kernel: the oracle's row-major list exactly, for random,
+    touching, identical, NaN and all-overlapping sets, sizes not multiples
+    of the wavefront or row block, a cap below the count, and empty sets."""
+    rs = np.random.RandomState(14)
+    a, b = _boxes(rs, 1001), _boxes(rs, 777, wmax=2.0)
+    a[:50, 0, 0] = b[:50, 0, 1]
+    a[50:100] = b[50:100]
+    a[100, 1, 0] = np.nan
+    b[7, 2, 1] = np.nan
+    o, _ = oracle.all_aabbs_overlap(a, b)
+    assert np.array_equal(kernels.overlap_dense(a, b).cpu().numpy(), o)
+    assert np.array_equal(kernels.overlap_dense(a, b, cap=10).cpu().numpy(), o)
+    # everything overlaps everything (a shared point)
+    c = np.zeros((701, 3, 2)); c[:, :, 0] = -rs.uniform(0.1, 1, (701, 3)); c[:, :, 1] = rs.uniform(0.1, 1, (701, 3))
+    d = np.zeros((333, 3, 2)); d[:, :, 0] = -rs.uniform(0.1, 1, (333, 3)); d[:, :, 1] = rs.uniform(0.1, 1, (333, 3))
+    g = kernels.overlap_dense(c, d).cpu().numpy()
+    assert len(g) == 701 * 333
+    assert np.array_equal(g, np.stack(np.meshgrid(np.arange(701), np.arange(333), indexing="ij"), -1).reshape(-1, 2))
+    assert len(kernels.overlap_dense(a[:0], b)) == 0 and len(kernels.overlap_dense(a, b[:0])) == 0
+
+
 # ---------------------------------- BoundingVolumeHierarchy drop-in
 
 class _TM:
