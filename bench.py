@@ -478,10 +478,16 @@ def run_contact(args, steps=10, warmup=2):
         ab1, X1, _ = kernels.tet_mesh_prep(T1)
         ab2, X2, _ = kernels.tet_mesh_prep(T2)
         pairs = kernels.overlap_dense(ab1, ab2, cap=6000000, counter=counter)
-        st = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, 1.0, 1.0, geometry=False)[0]
-        hp = pairs[torch.nonzero(st & 1).squeeze(1)]
-        st2, planes, nv, polys = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, hp, 1.0, 1.0)
-        F = kernels.tet_contact_forces(T1, e1, hp, st2, planes, nv, polys, 1.0)
+        st = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, 1.0, 1.0, geometry=False,
+                                     checked=False)[0]
+        rows = torch.nonzero(st & 0x101).squeeze(1)
+        sel = st[rows]
+        if bool((sel & 0x100).any()):
+            raise AssertionError("half-plane overflow")
+        hp = pairs[rows[(sel & 1) != 0]]
+        st2, planes, nv, polys = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, hp, 1.0, 1.0,
+                                                         checked=False)
+        F = kernels.tet_contact_forces(T1, e1, hp, st2, planes, nv, polys, 1.0, checked=False)
         out.update(pairs=pairs, hp=hp, planes=planes, nv=nv, polys=polys, F=F)
 
     wall, ev = timed_steps(step, steps, warmup)

@@ -65,23 +65,29 @@ def _check(status):
         raise AssertionError("more half-plane intersection points than the reference reserves")
 
 
-def _intersecting(T1, e1, X1, T2, e2, X2, pairs, E1, E2, forces):
+def _intersecting(T1, e1, X1, T2, e2, X2, pairs, E1, E2, forces, checked=True):
     """The intersecting pairs of a pair list, in list order, on the GPU: a
-    status-only pass over all pairs, the hit rows compacted on the device,
-    then planes, polygons (and forces) for the hits only.  Returns host
-    arrays (hit pairs (k,2), planes (k,4), n_vertices (k,), polygons
-    (k,24,3), forces (k,7) or None)."""
+    status-only pass over all pairs, the hit (and overflow) rows compacted on
+    the device, then planes, polygons (and forces) for the hits only.
+    Returns host arrays (hit pairs (k,2), planes (k,4), n_vertices (k,),
+    polygons (k,24,3), forces (k,7) or None).  ``checked=False``: the pairs
+    are known to be in range (produced by this library's broad phase)."""
     import torch
     pr = torch.as_tensor(pairs)
     pr = (pr if pr.is_cuda else pr.cuda()).to(torch.int32).reshape(-1, 2).contiguous()
-    st, _, _, _ = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, pr, E1, E2, geometry=False)
-    _check(np.array([int((st & OVERFLOW).max())]) if len(st) else np.zeros(1, dtype=np.int32))
-    hp = pr[torch.nonzero(st & 1).squeeze(1)]
+    st, _, _, _ = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, pr, E1, E2, geometry=False,
+                                          checked=checked)
+    rows = torch.nonzero(st & (1 | OVERFLOW)).squeeze(1)
+    sel = st[rows]
+    _check(sel.cpu().numpy())
+    hp = pr[rows[(sel & 1) != 0]]
     if hp.shape[0] == 0:
         return (np.empty((0, 2), dtype=np.int64), np.empty((0, 4)), np.empty(0, dtype=np.int32),
                 np.empty((0, kernels.TET_MAX_POLY, 3)), np.empty((0, 7)) if forces else None)
-    st2, planes, nv, polys = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, hp, E1, E2)
-    F = kernels.tet_contact_forces(T1, e1, hp, st2, planes, nv, polys, E1) if forces else None
+    st2, planes, nv, polys = kernels.tet_pairs_batch(T1, e1, X1, T2, e2, X2, hp, E1, E2,
+                                                     checked=False)
+    F = (kernels.tet_contact_forces(T1, e1, hp, st2, planes, nv, polys, E1, checked=False)
+         if forces else None)
     return (hp.cpu().numpy().astype(np.int64), planes.cpu().numpy(), nv.cpu().numpy(),
             polys.cpu().numpy(), F.cpu().numpy() if forces else None)
 
@@ -349,7 +355,7 @@ def find_contact_surface(rigid_body1, rigid_body2, use_aabb_trees=False):
         return surface
     hp, planes, nv, polys, F = _intersecting(
         T1, rigid_body1.tetrahedra_potentials, X1, T2, rigid_body2.tetrahedra_potentials, X2,
-        pairs, E1, E2, True)
+        pairs, E1, E2, True, checked=use_aabb_trees)
     k = len(hp)
     surface = ContactSurface(rigid_body2.body2origin_, k > 0, planes,
                              [polys[q, :nv[q]] for q in range(k)], hp[:, 0].tolist(),

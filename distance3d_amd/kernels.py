@@ -159,11 +159,13 @@ TET_MAX_POLY = 24
 
 
 def tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, youngs_modulus1=1.0, youngs_modulus2=1.0,
-                    stream=None, geometry=True):
+                    stream=None, geometry=True, checked=True):
     """Contact plane and polygon of tetrahedron pairs (d3d_tet_pairs_batch).
     T (n,4,3), e (n,4), X (n,4,4) per body, pairs (m,2); tensors or arrays.
     Returns device tensors (status (m,), planes (m,4), n_vertices (m,),
-    polygons (m,24,3)); planes and polygons are None when not ``geometry``."""
+    polygons (m,24,3)); planes and polygons are None when not ``geometry``.
+    ``checked=False`` skips the pair index range check (a device reduction
+    and a host read) for pairs produced in range by this library."""
     import torch
 
     def dev(a, shape):
@@ -176,8 +178,8 @@ def tet_pairs_batch(T1, e1, X1, T2, e2, X2, pairs, youngs_modulus1=1.0, youngs_m
     pr = torch.as_tensor(pairs, dtype=torch.int32)
     pr = (pr if pr.is_cuda else pr.cuda()).reshape(-1, 2).contiguous()
     m = pr.shape[0]
-    if m and (int(pr[:, 0].max()) >= T1.shape[0] or int(pr[:, 1].max()) >= T2.shape[0]
-              or int(pr.min()) < 0):
+    if checked and m and (int(pr[:, 0].max()) >= T1.shape[0] or int(pr[:, 1].max()) >= T2.shape[0]
+                          or int(pr.min()) < 0):
         raise ValueError("pair index out of range")
     status = torch.empty(m, dtype=torch.int32, device="cuda")
     planes = torch.empty((m, 4), dtype=torch.float64, device="cuda") if geometry else None
@@ -235,10 +237,12 @@ def tet_mesh_prep(T, aabbs=True, transforms=True, stream=None):
 
 
 def tet_contact_forces(T1, e1, pairs, status, planes, n_vertices, polygons, youngs_modulus1=1.0,
-                       stream=None):
+                       stream=None, checked=True):
     """Centre, force and area of each intersecting pair's contact polygon
     (d3d_tet_contact_forces); inputs as tet_pairs_batch takes and returns
-    them.  Returns a device tensor (m, 7): centre (3), force (3), area."""
+    them.  Returns a device tensor (m, 7): centre (3), force (3), area.
+    ``checked=False`` skips the index and vertex-count range checks for
+    inputs produced by tet_pairs_batch on in-range pairs."""
     import torch
     T1 = _dev64(T1, (-1, 4, 3))
     e1 = _dev64(e1, (-1, 4))
@@ -248,9 +252,9 @@ def tet_contact_forces(T1, e1, pairs, status, planes, n_vertices, polygons, youn
     pl = _dev64(planes, (m, 4))
     nv = _dev32(n_vertices, (m,))
     po = _dev64(polygons, (m, TET_MAX_POLY, 3))
-    if m and (int(pr[:, 0].max()) >= T1.shape[0] or int(pr[:, 0].min()) < 0):
+    if checked and m and (int(pr[:, 0].max()) >= T1.shape[0] or int(pr[:, 0].min()) < 0):
         raise ValueError("pair index out of range")
-    if m and int(nv.max()) > TET_MAX_POLY:
+    if checked and m and int(nv.max()) > TET_MAX_POLY:
         raise ValueError("polygon vertex count out of range")
     out = torch.empty((m, 7), dtype=torch.float64, device="cuda")
     check(lib().d3d_tet_contact_forces(ptr(T1), ptr(e1), ptr(pr), ptr(st), ptr(pl), ptr(nv), ptr(po),
