@@ -16,8 +16,10 @@ import numpy as np
 
 from . import kernels
 from .aabb_tree import AabbTree
+from .broad_phase import BoundingVolumeHierarchy
 from .tetra_mesh import (center_of_mass_tetrahedral_mesh, make_tetrahedral_box,
-                         make_tetrahedral_cube, make_tetrahedral_ellipsoid,
+                         make_tetrahedral_capsule, make_tetrahedral_cube,
+                         make_tetrahedral_cylinder, make_tetrahedral_ellipsoid,
                          make_tetrahedral_sphere, tetrahedral_mesh_volumes)
 
 tet_pairs_batch = kernels.tet_pairs_batch
@@ -211,6 +213,16 @@ class RigidBody:
         """_rigid_body.py:112-129."""
         return RigidBody(box2origin, *make_tetrahedral_box(size))
 
+    @staticmethod
+    def make_cylinder(cylinder2origin, radius, length, resolution_hint=0.1):
+        """_rigid_body.py:131-156."""
+        return RigidBody(cylinder2origin, *make_tetrahedral_cylinder(radius, length, resolution_hint))
+
+    @staticmethod
+    def make_capsule(capsule2origin, radius, height, resolution_hint=0.1):
+        """_rigid_body.py:158-184."""
+        return RigidBody(capsule2origin, *make_tetrahedral_capsule(radius, height, resolution_hint))
+
     @property
     def tetrahedra_points(self):
         """(m, 4, 3) points of every tetrahedron in the body frame."""
@@ -237,8 +249,14 @@ class RigidBody:
         self._invalidate()
 
     def update_pose(self, body2origin):
-        """_rigid_body.py:228-239."""
+        """_rigid_body.py:228-239 (the pose only; vertices stay where
+        express_in put them, as in the reference)."""
         self.body2origin_ = body2origin
+
+    @property
+    def artist_(self):
+        """No visualisation here (the reference's wireframe artist)."""
+        return None
 
     def aabb(self):
         """Box around the whole body."""
@@ -422,3 +440,14 @@ def contact_forces(rigid_body1, rigid_body2, return_details=False):
         details = surface.make_details(rigid_body1.tetrahedra_points, rigid_body2.tetrahedra_points)
         return surface.intersection, wrench12, wrench21, details
     return surface.intersection, wrench12, wrench21
+
+
+class HydroelasticBoundingVolumeHierarchy(BoundingVolumeHierarchy):
+    """Broad phase over RigidBody colliders (_broad_phase.py:7-98): the
+    BoundingVolumeHierarchy of this package with each body's box from its
+    tetrahedron boxes (RigidBody.aabb).  Bodies are added with add_collider;
+    filling from a URDF needs pytransform3d, which is not available."""
+
+    def aabb_overlapping_colliders(self, collider, whitelist=()):
+        """{frame: body} of the bodies whose box overlaps ``collider``'s."""
+        return super().aabb_overlapping_colliders(collider, whitelist)

@@ -154,6 +154,130 @@ def make_tetrahedral_box(size):
     return verts, np.array(tets, dtype=int), potentials
 
 
+def _split_prism(v0, v1, v2, v3, v4, v5):
+    """Three tetrahedra of the triangular prism (v0 v1 v2 | v3 v4 v5) around
+    the diagonal v0-v5 (_tetra_mesh_creation.py:468-474)."""
+    out, prev = [], v3
+    for nxt in (v4, v1, v2):
+        out.append([prev, nxt, v0, v5])
+        prev = nxt
+    return out
+
+
+def _split_pyramid(v0, v1, v2, v3, apex):
+    """Two tetrahedra of the pyramid over the quad (v0 v1 v2 v3)
+    (_tetra_mesh_creation.py:477-483)."""
+    return [[v3, apex, v0, v2], [apex, v1, v0, v2]]
+
+
+def make_tetrahedral_cylinder(radius, length, resolution_hint):
+    """Cylinder along z (_tetra_mesh_creation.py:302-465, Drake's cylinder
+    mesh): the two cap centres, then bottom/top rim vertices interleaved
+    (ceil(2 pi r / hint) per circle, at least 3), then the medial axis:
+    a segment (long: half length > r), a point (medium) or a circle of
+    radius r - L/2 at z = 0 plus the centre (short)."""
+    import math
+    top_z = 0.5 * length
+    tol = 1e-14 * max(1.0, min(top_z, radius))
+    n = max(3, math.ceil(2.0 * np.pi * radius / resolution_hint))
+    verts = [np.array([0.0, 0.0, -top_z]), np.array([0.0, 0.0, top_z])]
+    bc, tc = 0, 1
+    bottom, top = [], []
+    step = 2.0 * np.pi / n
+    for i in range(n):
+        x, y = radius * np.cos(step * i), radius * np.sin(step * i)
+        bottom.append(len(verts))
+        verts.append(np.array([x, y, -top_z]))
+        top.append(len(verts))
+        verts.append(np.array([x, y, top_z]))
+    pot = [0.0] * len(verts)
+    tets = []
+    if top_z - radius > tol:  # long: medial segment
+        m0, m1 = len(verts), len(verts) + 1
+        off = top_z - radius
+        verts += [np.array([0.0, 0.0, -off]), np.array([0.0, 0.0, off])]
+        pot += [radius, radius]
+        i = n - 1
+        for j in range(n):
+            tets.append([bc, bottom[i], bottom[j], m0])
+            tets.append([tc, top[j], top[i], m1])
+            tets.extend(_split_prism(m0, bottom[i], bottom[j], m1, top[i], top[j]))
+            i = j
+    elif radius - top_z > tol:  # short: medial circle and centre
+        centre = len(verts)
+        verts.append(np.array([0.0, 0.0, 0.0]))
+        pot.append(top_z)
+        scale = (radius - top_z) / radius
+        medial = []
+        for i in range(n):
+            medial.append(len(verts))
+            verts.append(np.array([verts[bottom[i]][0] * scale, verts[bottom[i]][1] * scale, 0.0]))
+            pot.append(top_z)
+        i = n - 1
+        for j in range(n):
+            tets.extend(_split_prism(bc, bottom[i], bottom[j], centre, medial[i], medial[j]))
+            tets.extend(_split_prism(centre, medial[i], medial[j], tc, top[i], top[j]))
+            tets.extend(_split_prism(bottom[i], medial[i], top[i], bottom[j], medial[j], top[j]))
+            i = j
+    else:  # medium: medial point at the centre
+        m = len(verts)
+        verts.append(np.array([0.0, 0.0, 0.0]))
+        pot.append(radius)
+        i = n - 1
+        for j in range(n):
+            tets.append([bc, bottom[i], bottom[j], m])
+            tets.append([tc, top[j], top[i], m])
+            tets.extend(_split_pyramid(top[i], top[j], bottom[j], bottom[i], m))
+            i = j
+    return np.array(verts), np.array(tets, dtype=int), np.array(pot)
+
+
+def make_tetrahedral_capsule(radius, height, resolution_hint):
+    """Capsule along z (_tetra_mesh_creation.py:486-593, Drake's capsule
+    mesh): medial top/bottom, the two poles, then the cap circles (top and
+    bottom interleaved, n = clip(2 pi r / hint, 3, 706) vertices each, n // 2
+    circles per cap from the equator up); pyramids between circles, fans at
+    the poles, prisms along the side around the medial segment."""
+    mz = 0.5 * height
+    pz = mz + radius
+    n = int(np.clip(2.0 * np.pi * radius / resolution_hint, 3.0, 706.0))
+    nc = n // 2
+    verts = [np.array([0.0, 0.0, mz]), np.array([0.0, 0.0, -mz]),
+             np.array([0.0, 0.0, pz]), np.array([0.0, 0.0, -pz])]
+    mtop, mbot, ptop, pbot = 0, 1, 2, 3
+    tcap, bcap = [], []
+    th_step = 0.5 * np.pi / nc
+    ph_step = 2.0 * np.pi / n
+    for i in range(nc):
+        theta = 0.5 * np.pi - i * th_step
+        st = np.sin(theta)
+        z = radius * np.cos(theta) + mz
+        for j in range(n):
+            phi = j * ph_step
+            x, y = radius * st * np.cos(phi), radius * st * np.sin(phi)
+            tcap.append(len(verts))
+            verts.append(np.array([x, y, z]))
+            bcap.append(len(verts))
+            verts.append(np.array([x, y, -z]))
+    tets = []
+    for i in range(nc - 1):
+        for j in range(n):
+            j1 = (j + 1) % n
+            tets.extend(_split_pyramid(tcap[(i + 1) * n + j], tcap[(i + 1) * n + j1],
+                                       tcap[i * n + j1], tcap[i * n + j], mtop))
+            tets.extend(_split_pyramid(bcap[i * n + j], bcap[i * n + j1],
+                                       bcap[(i + 1) * n + j1], bcap[(i + 1) * n + j], mbot))
+    last = (nc - 1) * n
+    for j in range(n):
+        j1 = (j + 1) % n
+        tets.append([ptop, tcap[last + j1], tcap[last + j], mtop])
+        tets.append([pbot, bcap[last + j], bcap[last + j1], mbot])
+        tets.extend(_split_prism(mbot, bcap[j], bcap[j1], mtop, tcap[j], tcap[j1]))
+    pot = np.zeros(len(verts))
+    pot[:2] = radius
+    return np.array(verts), np.array(tets, dtype=int), pot
+
+
 def tetrahedral_mesh_volumes(tetrahedra_points):
     """|((p1 - p0) x (p2 - p0)) . (p3 - p0)| / 6 per tetrahedron
     (_mesh_processing.py:43-60)."""

@@ -222,6 +222,34 @@ def test_mesh_topology():
         assert np.all(p[:8] == 0.0) and np.all(p[8:] == min(size) / 2)
 
 
+def test_cylinder_and_capsule_meshes():
+    """Long, medium and short cylinders fill the inscribed polygonal prism
+    exactly (volume n/2 r^2 sin(2 pi/n) L, no degenerate tetrahedra); the
+    capsule approaches the round capsule's volume; centres of mass at the
+    origin (test_hydroelastic_contact.py:139-253)."""
+    import math
+    for r, L, hint in ((0.1, 1.0, 0.1), (0.1, 0.2, 0.1), (0.1, 0.1, 0.1), (0.1, 1.0, 0.01),
+                       (0.3, 0.2, 0.05), (0.2, 0.1, 0.01)):
+        v, t, p = tetra_mesh.make_tetrahedral_cylinder(r, L, hint)
+        n = max(3, math.ceil(2.0 * np.pi * r / hint))
+        V = tetra_mesh.tetrahedral_mesh_volumes(v[t])
+        assert V.min() > 1e-12 * r * r * L
+        assert np.sum(V) == pytest.approx(0.5 * n * r * r * np.sin(2 * np.pi / n) * L, rel=1e-12)
+        np.testing.assert_array_almost_equal(tetra_mesh.center_of_mass_tetrahedral_mesh(v[t]), np.zeros(3))
+        assert np.all(p[:2 + 2 * n] == 0.0) and np.all(p[2 + 2 * n:] > 0.0)
+    for r, h, hint in ((0.1, 1.0, 0.01), (0.1, 0.0, 0.01), (0.2, 0.5, 0.02)):
+        v, t, p = tetra_mesh.make_tetrahedral_capsule(r, h, hint)
+        V = tetra_mesh.tetrahedral_mesh_volumes(v[t])
+        # height 0: the medial segment is a point and the side prisms are
+        # flat, as the reference builds them
+        assert V.min() > 0.0 if h > 0.0 else V.min() >= 0.0
+        assert np.sum(V) == pytest.approx(np.pi * r * r * h + 4.0 / 3.0 * np.pi * r ** 3, rel=2e-2)
+        np.testing.assert_array_almost_equal(tetra_mesh.center_of_mass_tetrahedral_mesh(v[t]), np.zeros(3))
+    rb = hc.RigidBody.make_cylinder(np.eye(4), 0.1, 1.0, resolution_hint=0.01)
+    assert np.sum(tetra_mesh.tetrahedral_mesh_volumes(rb.tetrahedra_points)) == pytest.approx(
+        np.pi * 0.01 * 1.0, abs=1e-2)
+
+
 def test_rigid_body_transforms():  # test_hydroelastic_contact.py:67-86
     cube12origin = np.eye(4)
     cube1_22origin = np.copy(cube12origin)
@@ -357,6 +385,16 @@ def test_contact_forces_dropin_golden(gpu):
 
 
 @pytest.mark.gpu
+def test_cylinder_tetrahedron_with_itself(gpu):  # test_hydroelastic_contact.py:89-98
+    cyl = hc.RigidBody.make_cylinder(np.eye(4), 0.1, 1.0, resolution_hint=0.1)
+    tet = cyl.tetrahedra_points[0]
+    eps = np.array([0.0, 0.0, 0.0, 1.0])
+    X = hc.barycentric_transforms(tet.reshape(1, 4, 3))[0]
+    hit, _ = hc.intersect_tetrahedron_pair(tet, eps, X, tet, eps, X)
+    assert hit
+
+
+@pytest.mark.gpu
 def test_single_polygon_and_mesh_helpers(gpu):
     """compute_contact_force on the same-tetrahedron case
     (test_hydroelastic_contact.py:101-116) and on random polygons against the
@@ -386,3 +424,50 @@ def test_single_polygon_and_mesh_helpers(gpu):
     ab = hc.tetrahedral_mesh_aabbs(rb.tetrahedra_points)
     assert np.array_equal(ab[:, :, 0], rb.tetrahedra_points.min(axis=1))
     assert np.array_equal(ab[:, :, 1], rb.tetrahedra_points.max(axis=1))
+
+
+class _Frames:
+    """Poses of frames in "origin" (the transform-manager calls the BVH makes)."""
+
+    def __init__(self):
+        self.T = {}
+
+    def add_transform(self, a, b, A2B):
+        self.T[a] = np.asarray(A2B, dtype=float)
+
+    def get_transform(self, a, b):
+        return self.T[a]
+
+
+@pytest.mark.gpu
+def test_hydroelastic_bvh(gpu):
+    """_broad_phase.py:7-98: body boxes from the tetrahedron boxes; overlap
+    queries equal brute force over those boxes."""
+    rs = np.random.RandomState(15)
+    tm = _Frames()
+    bvh = hc.HydroelasticBoundingVolumeHierarchy(tm, "base")
+    bodies = {}
+    for k in range(12):
+        P = pose(rs, rs.uniform(-1.0, 1.0, 3))
+        tm.add_transform("b%d" % k, "origin", P)
+        body = (hc.RigidBody.make_sphere(P[:3, 3], rs.uniform(0.1, 0.5), 1) if k % 2 else
+                hc.RigidBody.make_box(P, rs.uniform(0.2, 0.8, 3)))
+        body.express_in(np.eye(4))
+        bodies["b%d" % k] = body
+        bvh.add_collider("b%d" % k, body)
+    boxes = {f: b.aabb() for f, b in bodies.items()}
+    for f, b in bodies.items():
+        pts = b.tetrahedra_points.reshape(-1, 3)
+        assert np.array_equal(boxes[f][:, 0], pts.min(axis=0))
+        assert np.array_equal(boxes[f][:, 1], pts.max(axis=0))
+
+    def overlap(a, b):
+        return bool(np.all(a[:, 0] <= b[:, 1]) and np.all(a[:, 1] >= b[:, 0]))
+
+    for f, b in bodies.items():
+        got = set(bvh.aabb_overlapping_colliders(b, whitelist=(f,)))
+        want = {g for g in bodies if g != f and overlap(boxes[g], boxes[f])}
+        assert got == want
+    pairs = {tuple(sorted((p[0][0], p[1][0]))) for p in bvh.aabb_overlapping_with_self()}
+    want = {tuple(sorted((f, g))) for f in bodies for g in bodies if f != g and overlap(boxes[f], boxes[g])}
+    assert pairs == want and len(want) > 0
