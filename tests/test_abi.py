@@ -36,3 +36,31 @@ def test_library_exports_declared_symbols():
 
 def test_binding_covers_header():
     assert _declared() == set(_native.exported_symbols())
+
+
+def test_argument_errors_without_gpu():
+    """Bad arguments are rejected with D3D_E_INVALID and a message before any
+    HIP call, and empty inputs return D3D_OK without one (runs with no GPU)."""
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.fail("libd3d_hip.so not built; run __graft_entry__.build()")
+    lib = _native.load_library()
+    invalid = 1
+    assert lib.d3d_tet_mesh_prep(None, 5, None, None, None, None) == invalid
+    assert b"NULL" in lib.d3d_last_error()
+    assert lib.d3d_tet_mesh_prep(None, -1, None, None, None, None) == invalid
+    assert lib.d3d_tet_mesh_prep(None, 0, None, None, None, None) == 0
+    assert lib.d3d_tet_contact_forces(None, None, None, None, None, None, None, 4, 1.0, None,
+                                      None) == invalid
+    assert lib.d3d_tet_contact_forces(None, None, None, None, None, None, None, 0, 1.0, None,
+                                      None) == 0
+    assert lib.d3d_polygon_angles(None, None, 3, None, None) == invalid
+    assert lib.d3d_polygon_angles(None, None, 0, None, None) == 0
+    assert lib.d3d_overlap_dense(None, 3, None, 3, None, 0, None, None, 0, None) == invalid
+    assert lib.d3d_overlap_dense(None, -1, None, 3, None, 0, None, None, 0, None) == invalid
+    # workspace sizes: pure host arithmetic
+    assert lib.d3d_overlap_dense_workspace_bytes(-1) == -1
+    assert lib.d3d_overlap_dense_workspace_bytes(5) == 48
+    assert lib.d3d_tet_pairs_workspace_bytes(-1) == -1
+    assert lib.d3d_tet_pairs_workspace_bytes(0) == 0
+    assert lib.d3d_tet_pairs_workspace_bytes(100) == 128 * 48 * 8
+    assert lib.d3d_tet_pairs_workspace_bytes(1 << 30) == 4096 * 64 * 48 * 8
