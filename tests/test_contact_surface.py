@@ -471,3 +471,35 @@ def test_hydroelastic_bvh(gpu):
     pairs = {tuple(sorted((p[0][0], p[1][0]))) for p in bvh.aabb_overlapping_with_self()}
     want = {tuple(sorted((f, g))) for f in bodies for g in bodies if f != g and overlap(boxes[f], boxes[g])}
     assert pairs == want and len(want) > 0
+
+
+@pytest.mark.gpu
+def test_empty_and_disjoint_inputs(gpu):
+    """Bodies far apart (no candidate pairs), candidate pairs with no hit,
+    and empty arrays through every new wrapper."""
+    from distance3d_amd import kernels
+    rb1 = hc.RigidBody.make_sphere(np.array([0.0, 0.0, 0.0]), 0.15, 1)
+    rb2 = hc.RigidBody.make_sphere(np.array([5.0, 0.0, 0.0]), 0.15, 1)
+    s = hc.find_contact_surface(rb1, rb2)
+    assert not s.intersection and len(s.contact_polygons) == 0 and s.contact_planes.shape == (0, 4)
+    hit, w12, w21 = hc.contact_forces(hc.RigidBody.make_sphere(np.zeros(3), 0.15, 1),
+                                      hc.RigidBody.make_sphere(np.array([5.0, 0, 0]), 0.15, 1))
+    assert not hit and np.all(w12 == 0.0) and np.all(w21 == 0.0)
+    assert hc.contact_forces(hc.RigidBody.make_sphere(np.zeros(3), 0.15, 1),
+                             hc.RigidBody.make_sphere(np.array([5.0, 0, 0]), 0.15, 1),
+                             return_details=True)[3] == {}
+    # pairs of tetrahedra far apart: no contact polygon
+    rs = np.random.RandomState(16)
+    T = random_tets(rs, 50)
+    z = rs.uniform(0.1, 1.0, (50, 4))
+    X = hc.barycentric_transforms(T)
+    res = hc.intersect_tetrahedron_pairs([(i, i) for i in range(50)], T, T + 10.0, z, z, X,
+                                         hc.barycentric_transforms(T + 10.0))
+    assert res[0] is False and res[3] == [] and res[4] == []
+    assert hc.intersect_tetrahedron_pairs([], T, T, z, z, X, X) == (False, [], [], [], [])
+    ab, Xe, fl = kernels.tet_mesh_prep(np.empty((0, 4, 3)))
+    assert ab.shape == (0, 3, 2) and Xe.shape == (0, 4, 4) and fl.shape == (0,)
+    assert kernels.polygon_angles(np.empty(0), np.empty(0)).shape == (0,)
+    out = kernels.tet_contact_forces(T, z, np.empty((0, 2)), np.empty(0), np.empty((0, 4)),
+                                     np.empty(0), np.empty((0, kernels.TET_MAX_POLY, 3)))
+    assert out.shape == (0, 7)
