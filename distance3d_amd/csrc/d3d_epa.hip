@@ -41,35 +41,46 @@ struct EpaLds {
     // copies (-1: not cached)
     float cx[kEpaCache], cy[kEpaCache], cz[kEpaCache];
     int coff[2];
+    int split;  // both hulls cached: A and B swept by the two halves at once
 };
 
 // Argmax over a hull whose FP32 vertices are cached in LDS at `off` (pool
-// order), by the 64 lanes of the wavefront: the FP32 sweep and certificate of
-// hull_argmax, with its FP64 rescan from the pools when the certificate fails.
+// order), by the G lanes of a group (G = 64: the wavefront; G = 32: one half,
+// the other half sweeping another hull at the same time; `sub` = lane index
+// in the group): the FP32 sweep and certificate of hull_argmax, with its FP64
+// rescan from the pools when the certificate fails.  All 64 lanes call it.
+template <int G>
 __device__ __forceinline__ int lds_hull_argmax(const d3d_scene &s, const EpaLds &L, int off,
-                                               const HullRef &H, v3 l) {
-    const int lane = lane_id();
+                                               const HullRef &H, v3 l, int sub) {
     const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
     const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
     const double E = 8.0 * 5.9604644775390625e-08 * H.L * linf;
     float m1 = -INFINITY, m2 = -INFINITY;
     int i1 = 0x7fffffff;
-    for (int j = lane; j < H.nv; j += kWave)
+    for (int j = sub; j < H.nv; j += G)
         top2(__builtin_fmaf(L.cz[off + j], dz, __builtin_fmaf(L.cy[off + j], dy, L.cx[off + j] * dx)), j,
              m1, m2, i1);
-    group_top2<kWave>(m1, m2, i1);
-    if ((double)m1 - (double)m2 > 2.0 * E) return i1;
+    group_top2<G>(m1, m2, i1);
+    const bool certified = (double)m1 - (double)m2 > 2.0 * E;
+    if (__ballot(!certified) == 0ull) return i1;
     // the FP64 rescan of hull_rescan, with the candidates found in the cache
+    // (groups whose certificate held take no candidate and keep i1)
     float thr = __double2float_rd((double)m1 - 2.5 * E);
     if (thr != thr) thr = -INFINITY;
     ArgmaxF64 A;
     A.init();
-    for (int j = lane; j < H.nv; j += kWave) {
-        const float sc =
-            __builtin_fmaf(L.cz[off + j], dz, __builtin_fmaf(L.cy[off + j], dy, L.cx[off + j] * dx));
-        if (!(sc < thr)) rescan_take(s, H, l, j, A);
-    }
-    return A.group_result<kWave>();
+    if (!certified)
+        for (int j = sub; j < H.nv; j += G) {
+            const float sc =
+                __builtin_fmaf(L.cz[off + j], dz, __builtin_fmaf(L.cy[off + j], dy, L.cx[off + j] * dx));
+            if (!(sc < thr)) rescan_take(s, H, l, j, A);
+        }
+    const int r = A.group_result<G>();
+    return certified ? i1 : r;
+}
+
+__device__ __forceinline__ v3 readlane_v3(v3 v, int lane) {
+    return mk(readlane_f64(v.x, lane), readlane_f64(v.y, lane), readlane_f64(v.z, lane));
 }
 
 // Supports of record ia along d and of record ib along -d for the EPA
@@ -78,6 +89,21 @@ __device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, cons
                                                         int ib, v3 d) {
     v3 a = mk(0.0, 0.0, 0.0), b = a;
     d = mk(wave_uniform(d.x), wave_uniform(d.y), wave_uniform(d.z));
+    if (L.split) {
+        // both hulls cached: lanes 0..31 sweep A along d, lanes 32..63 sweep
+        // B along -d, at the same time; the supports are read from lanes 0
+        // and 32
+        const int half = lane_id() >> 5;
+        const Shape h = load_shape(s, half ? ib : ia);
+        const v3 dir = vsel(half == 0, d, -d);
+        const v3 l = to_local(h, dir);
+        const int off = L.coff[half];
+        const int idx = lds_hull_argmax<32>(s, L, off, hull_ref(h), l, lane_id() & 31);
+        const v3 v = h.voff64 < 0 ? mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx])
+                                  : hull_vertex(s, h.voff, h.voff64, idx);
+        const v3 out = apply_margin(h, to_world(h, v), dir);
+        return SupportPair{readlane_v3(out, 0), readlane_v3(out, 32)};
+    }
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
         const Shape h = load_shape(s, wave_uniform(k == 0 ? ia : ib));
@@ -89,7 +115,7 @@ __device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, cons
             const int off = L.coff[k];
             v3 v;
             if (off >= 0) {
-                const int idx = lds_hull_argmax(s, L, off, H, l);
+                const int idx = lds_hull_argmax<kWave>(s, L, off, H, l, lane_id());
                 v = h.voff64 < 0 ? mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx])
                                  : hull_vertex(s, h.voff, h.voff64, idx);
             } else {
@@ -298,6 +324,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             if (lane == 0) {
                 L.coff[0] = fit && na > 0 ? 0 : -1;
                 L.coff[1] = fit && nb > 0 ? na : -1;
+                L.split = fit && na > 0 && nb > 0;
             }
         }
         // ---- seed: the GJK simplex rows (w, support on A, support on B) go
