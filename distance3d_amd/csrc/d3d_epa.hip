@@ -42,7 +42,28 @@ struct EpaLds {
     float cx[kEpaCache], cy[kEpaCache], cz[kEpaCache];
     int coff[2];
     int split;  // both hulls cached: A and B swept by the two halves at once
+    // the two collider records (scene layout), staged once per pair
+    double recf[2][16];
+    int32_t reci[2][4];
 };
+
+// Collider record k (0: A, 1: B) of the wavefront's pair from LDS.
+__device__ __forceinline__ Shape lds_shape(const EpaLds &L, int k) {
+    Shape h;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) h.R[j] = L.recf[k][j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        h.t[j] = L.recf[k][9 + j];
+        h.p[j] = L.recf[k][12 + j];
+    }
+    h.margin = L.recf[k][15];
+    h.type = L.reci[k][0];
+    h.nv = L.reci[k][1];
+    h.voff = L.reci[k][2];
+    h.voff64 = L.reci[k][3];
+    return h;
+}
 
 // Argmax over a hull whose FP32 vertices are cached in LDS at `off` (pool
 // order), by the G lanes of a group (G = 64: the wavefront; G = 32: one half,
@@ -83,10 +104,10 @@ __device__ __forceinline__ v3 readlane_v3(v3 v, int lane) {
     return mk(readlane_f64(v.x, lane), readlane_f64(v.y, lane), readlane_f64(v.z, lane));
 }
 
-// Supports of record ia along d and of record ib along -d for the EPA
-// wavefront (see support_pair), sweeping LDS-cached hulls from LDS.
-__device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, const EpaLds &L, int ia,
-                                                        int ib, v3 d) {
+// Supports of the pair's record A along d and of record B along -d for the
+// EPA wavefront (see support_pair): the records come from LDS, staged once per
+// pair, and LDS-cached hulls are swept from LDS.
+__device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, const EpaLds &L, v3 d) {
     v3 a = mk(0.0, 0.0, 0.0), b = a;
     d = mk(wave_uniform(d.x), wave_uniform(d.y), wave_uniform(d.z));
     if (L.split) {
@@ -94,7 +115,7 @@ __device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, cons
         // B along -d, at the same time; the supports are read from lanes 0
         // and 32
         const int half = lane_id() >> 5;
-        const Shape h = load_shape(s, half ? ib : ia);
+        const Shape h = lds_shape(L, half);
         const v3 dir = vsel(half == 0, d, -d);
         const v3 l = to_local(h, dir);
         const int off = L.coff[half];
@@ -106,7 +127,7 @@ __device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, cons
     }
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
-        const Shape h = load_shape(s, wave_uniform(k == 0 ? ia : ib));
+        const Shape h = lds_shape(L, k);
         const v3 dir = vsel(k == 0, d, -d);
         v3 out;
         if (h.type == D3D_HULL) {
@@ -196,7 +217,7 @@ __device__ __forceinline__ void put_row(EpaLds &L, int r, v3 w, v3 a, v3 b) {
 // the higher side.  One support call site (the loop below) keeps a single
 // copy of the sweep; candidates go straight to their LDS rows (n == 3 uses
 // row 4 as scratch).  Returns false for a degenerate seed.
-__device__ bool epa_seed(const d3d_scene &s, int ia, int ib, EpaLds &L, int n) {
+__device__ bool epa_seed(const d3d_scene &s, EpaLds &L, int n) {
     if (n < 1 || n > 4) return false;
     while (n < 4) {
         const v3 w0 = row_w(L, 0);
@@ -226,7 +247,7 @@ __device__ bool epa_seed(const d3d_scene &s, int ia, int ib, EpaLds &L, int n) {
             } else {
                 d = vsel(k == 0, nrm, -nrm);
             }
-            const SupportPair sp = epa_support_pair(s, L, ia, ib, d);
+            const SupportPair sp = epa_support_pair(s, L, d);
             const v3 w = sp.a - sp.b;
             if (n == 3) {
                 put_row(L, 3 + k, w, sp.a, sp.b);
@@ -304,6 +325,12 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             continue;
         }
 
+        // ---- the two collider records into LDS
+        {
+            const int64_t ra = pa[pid], rb = pb[pid];
+            if (lane < 32) L.recf[lane >> 4][lane & 15] = s.shape_f64[16 * (lane < 16 ? ra : rb) + (lane & 15)];
+            else if (lane < 40) L.reci[(lane - 32) >> 2][lane & 3] = s.shape_i32[4 * (lane < 36 ? ra : rb) + (lane & 3)];
+        }
         // ---- hull vertices into the LDS cache when both fit
         {
             const int na = SA.type == D3D_HULL ? SA.nv : 0;
@@ -339,8 +366,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
         }
         __syncthreads();
         int status = 0;
-        const int ia = pa[pid], ib = pb[pid];
-        bool seed_ok = epa_seed(s, ia, ib, L, npts[pid]);
+        bool seed_ok = epa_seed(s, L, npts[pid]);
         int nf = 0;
         if (seed_ok) {
             const v3 w0 = row_w(L, 0);
@@ -402,7 +428,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             kb = bk;
             const v3 cn = mk(L.fnx[kb], L.fny[kb], L.fnz[kb]);
             const double cdist = L.fd[kb];
-            const SupportPair sp = epa_support_pair(s, L, ia, ib, cn);
+            const SupportPair sp = epa_support_pair(s, L, cn);
             const v3 spa = sp.a, spb = sp.b;
             const v3 w = spa - spb;
             const double wn = dot(w, cn);
