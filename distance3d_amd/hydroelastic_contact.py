@@ -106,6 +106,13 @@ def intersect_tetrahedron_pairs(pairs, tetrahedra_points1, tetrahedra_points2, e
     pr = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
     if len(pr) == 0:
         return False, [], [], [], []
+    # the reference looks X up per pair (X1[i], X2[j]): a missing entry of a
+    # dict is a KeyError there too
+    for X, col in ((X1, 0), (X2, 1)):
+        if isinstance(X, dict):
+            missing = set(np.unique(pr[:, col]).tolist()) - set(X)
+            if missing:
+                raise KeyError(min(missing))
     hp, planes, nv, polys, _ = _intersecting(T1, epsilon1, _dense(X1, len(T1)), T2, epsilon2,
                                              _dense(X2, len(T2)), pr, youngs_modulus1,
                                              youngs_modulus2, False)

@@ -250,6 +250,19 @@ def test_cylinder_and_capsule_meshes():
         np.pi * 0.01 * 1.0, abs=1e-2)
 
 
+def test_missing_transform_is_a_key_error():
+    """Transforms given as dicts are looked up per pair, as the reference does
+    (_tetrahedron_intersection.py:65-66): a missing entry raises KeyError
+    before any GPU work."""
+    T = random_tets(np.random.RandomState(17), 3)
+    e = np.ones((3, 4))
+    X = {0: np.eye(4), 2: np.eye(4)}
+    with pytest.raises(KeyError):
+        hc.intersect_tetrahedron_pairs([(0, 0), (1, 2)], T, T, e, e, X, X)
+    with pytest.raises(KeyError):
+        hc.intersect_tetrahedron_pairs([(0, 0), (2, 1)], T, T, e, e, X, X)
+
+
 def test_rigid_body_transforms():  # test_hydroelastic_contact.py:67-86
     cube12origin = np.eye(4)
     cube1_22origin = np.copy(cube12origin)
