@@ -90,6 +90,7 @@ _SIGNATURES = {
                             _P],
     "d3d_tet_mesh_prep": [_P, _I64, _P, _P, _P, _P],
     "d3d_polygon_angles": [_P, _P, _I64, _P, _P],
+    "d3d_intersect_halfplanes": [_P, _I32, _P, _P, _P],
     "d3d_tet_contact_forces": [_P, _P, _P, _P, _P, _P, _P, _I64, _D, _P, _P],
     "d3d_cluster_plan": [_P, _I64, _I32, _P, _P, _P],
     "d3d_cluster_scene": [_P, _P, _I64, _P, _P, _P, _P, _I64, _I64, _I32, _I32,

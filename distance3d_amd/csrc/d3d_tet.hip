@@ -445,6 +445,54 @@ __global__ __launch_bounds__(kTetBlock) void tet_force_kernel(
     o[6] = ta;
 }
 
+// intersect_halfplanes (_halfplanes.py:29-64) for one set of n half-planes
+// (rows p, pq) by one wavefront: pairs (i, j > i) in the reference's loop
+// order, lanes taking consecutive j; valid points land in that order (ballot
+// ranks).  count = the number of points, or -1 where the reference asserts
+// (a valid point past the 3n - 1 it has room for).
+__global__ __launch_bounds__(kWave) void halfplane_kernel(const double *__restrict__ hp, int n,
+                                                         double *__restrict__ pts,
+                                                         int32_t *__restrict__ count) {
+    const int lane = threadIdx.x;
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+    int cnt = 0;
+    bool overflow = false;
+    for (int i = 0; i < n && !overflow; ++i) {
+        const double *a = hp + 4 * i;
+        for (int j0 = i + 1; j0 < n && !overflow; j0 += kWave) {
+            const int j = j0 + lane;
+            bool valid = false;
+            double px = 0.0, py = 0.0;
+            if (j < n) {
+                const double *b = hp + 4 * j;
+                const double denom = a[2] * b[3] - a[3] * b[2];
+                if (!(fabs(denom) < kEps)) {
+                    const double q0 = b[0] - a[0], q1 = b[1] - a[1];
+                    const double t = (q0 * b[3] - q1 * b[2]) / denom;
+                    px = a[0] + a[2] * t;
+                    py = a[1] + a[3] * t;
+                    valid = true;
+                    for (int k = 0; k < n && valid; ++k) {
+                        if (k == i || k == j) continue;
+                        const double *c = hp + 4 * k;
+                        const double r0 = px - c[0], r1 = py - c[1];
+                        if (c[2] * r1 - c[3] * r0 < -kEps) valid = false;
+                    }
+                }
+            }
+            const unsigned long long m = __ballot(valid);
+            const int pos = cnt + __popcll(m & lt);
+            if (valid && pos + 1 < 3 * n) {
+                pts[2 * pos] = px;
+                pts[2 * pos + 1] = py;
+            }
+            overflow = __ballot(valid && pos + 1 >= 3 * n) != 0ull;
+            cnt += __popcll(m);
+        }
+    }
+    if (lane == 0) *count = overflow ? -1 : cnt;
+}
+
 __global__ __launch_bounds__(kTetBlock) void angle_kernel(const double *__restrict__ y,
                                                           const double *__restrict__ x, int64_t n,
                                                           double *__restrict__ out) {
@@ -524,5 +572,16 @@ extern "C" int d3d_polygon_angles(const double *y, const double *x, int64_t n, d
     hipLaunchKernelGGL(angle_kernel, dim3(grid_for(n, kTetBlock)), dim3(kTetBlock), 0,
                        (hipStream_t)stream, y, x, n, out);
     D3D_CHECK_LAUNCH("angle_kernel");
+    return D3D_OK;
+}
+
+extern "C" int d3d_intersect_halfplanes(const double *halfplanes, int32_t n, double *points,
+                                        int32_t *count, void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(n >= 0 && n <= (1 << 20), "n must be in [0, 2^20]");
+    D3D_CHECK_ARG(count && (n == 0 || (halfplanes && points)), "NULL buffer");
+    hipLaunchKernelGGL(halfplane_kernel, dim3(1), dim3(kWave), 0, (hipStream_t)stream, halfplanes,
+                       n, points, count);
+    D3D_CHECK_LAUNCH("halfplane_kernel");
     return D3D_OK;
 }

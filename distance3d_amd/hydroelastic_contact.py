@@ -120,6 +120,24 @@ def intersect_tetrahedron_pairs(pairs, tetrahedra_points1, tetrahedra_points2, e
             hp[:, 0].tolist(), hp[:, 1].tolist())
 
 
+def intersect_halfplanes(halfplanes):
+    """Polygon points of a set of half-planes, rows (p, pq)
+    (_halfplanes.py:29-64), on the GPU (d3d_intersect_halfplanes); raises
+    AssertionError where the reference does."""
+    import torch
+    hp = torch.as_tensor(np.ascontiguousarray(halfplanes, dtype=np.float64).reshape(-1, 4)).cuda()
+    n = hp.shape[0]
+    pts = torch.empty((max(3 * n, 1), 2), dtype=torch.float64, device="cuda")
+    count = torch.empty(1, dtype=torch.int32, device="cuda")
+    kernels.check(kernels.lib().d3d_intersect_halfplanes(kernels.ptr(hp), n, kernels.ptr(pts),
+                                                         kernels.ptr(count), kernels.stream_ptr(None)),
+                  "d3d_intersect_halfplanes")
+    k = int(count.item())
+    if k < 0:
+        raise AssertionError("more half-plane intersection points than the reference reserves")
+    return pts[:k].cpu().numpy()
+
+
 def intersect_tetrahedron_pair(tetrahedron1, epsilon1, X1, tetrahedron2, epsilon2, X2,
                                youngs_modulus1=1.0, youngs_modulus2=1.0):
     """Intersect a pair of tetrahedra (_tetrahedron_intersection.py:89-139):

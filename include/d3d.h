@@ -320,6 +320,16 @@ int d3d_tet_pairs_batch(const double *T1, const double *e1, const double *X1, co
 int d3d_polygon_angles(const double *y, const double *x, int64_t n, double *out, void *stream);
 
 /*
+ * intersect_halfplanes (hydroelastic_contact/_halfplanes.py:29-64): the
+ * pairwise intersections of n half-planes (rows p.x, p.y, pq.x, pq.y, f64)
+ * that lie inside all the others, in the reference's (i, j > i) order, to
+ * points (3n, 2); *count (device int32) = their number, or -1 where the
+ * reference asserts (more points than it reserves).  One wavefront.
+ */
+int d3d_intersect_halfplanes(const double *halfplanes, int32_t n, double *points, int32_t *count,
+                             void *stream);
+
+/*
  * Per-tetrahedron preparation of a tetrahedral mesh (T (n,4,3) f64): its
  * AABBs (n,3,2) [axis][min,max] (tetrahedral_mesh_aabbs,
  * hydroelastic_contact/_mesh_processing.py:4-20) and barycentric transforms
