@@ -47,7 +47,7 @@ class d3d_scene(ctypes.Structure):
     ]
 
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 CLUSTER_SIZE = 16
 
 
@@ -77,7 +77,7 @@ _SIGNATURES = {
     "d3d_gjk_distance_batch_ws": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params),
                                   _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
     "d3d_gjk_intersection_batch": [_SCENE, _P, _P, _I64, _D, _I32, _P, _P, _P],
-    "d3d_epa_batch": [_SCENE, _P, _P, _I64, _P, _P, _I32, _I32, _D, _P, _P, _P, _P],
+    "d3d_epa_batch": [_SCENE, _P, _P, _I64, _P, _P, _I32, _I32, _D, _P, _P, _P, _P, _P, _P],
     "d3d_gjk_epa_batch_ws": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params), _I32, _I32,
                              _D, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
     "d3d_instance_pairs": [_SCENE, _P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P],

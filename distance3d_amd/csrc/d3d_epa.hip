@@ -21,8 +21,12 @@
 namespace d3d {
 
 constexpr int kEpaMaxFaces = 128;
-constexpr int kEpaMaxIter = 64;
-constexpr int kEpaMaxVerts = kEpaMaxIter + 8;
+// polytope vertex rows: the 4 seed rows (plus scratch) and one per iteration;
+// when they run out, rows no face refers to are recycled (epa_compact_rows).
+// A closed triangulated polytope with F faces has F/2 + 2 vertices (66 for
+// 128 faces), so recycling always frees rows unless the polytope is not a
+// closed surface; then the loop ends like the face cap (status 2).
+constexpr int kEpaMaxVerts = 72;
 // FP32 vertex cache: a pair whose hulls have at most this many vertices in
 // total is swept from LDS (most EPA pairs; larger ones use the global sweep)
 constexpr int kEpaCache = 672;
@@ -282,13 +286,70 @@ __device__ bool epa_seed(const d3d_scene &s, EpaLds &L, int n) {
     return true;
 }
 
+// Drops the polytope's vertex rows that no face refers to any more (vertices
+// buried by earlier expansions) and renumbers the others in order, remapping
+// the faces.  Only row numbers change, so the expansion goes on exactly as it
+// would with more rows; a polytope of at most 128 faces refers to at most 66
+// vertices.  The edge bitmap is all zero between iterations; its first three
+// words hold the reference mask meanwhile.  Returns the new row count.
+__device__ int epa_compact_rows(EpaLds &L, int nv, int nf, int lane) {
+    static_assert(kEpaMaxVerts <= 96, "reference mask holds 96 rows");
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl) {
+        const int k = sl * kWave + lane;
+        if (k < nf) {
+            const int v0 = L.fv0[k], v1 = L.fv1[k], v2 = L.fv2[k];
+            atomicOr(&L.eb[v0 >> 5], 1u << (v0 & 31));
+            atomicOr(&L.eb[v1 >> 5], 1u << (v1 & 31));
+            atomicOr(&L.eb[v2 >> 5], 1u << (v2 & 31));
+        }
+    }
+    __syncthreads();
+    const unsigned m0 = L.eb[0], m1 = L.eb[1], m2 = L.eb[2];
+    __syncthreads();
+    if (lane < 3) L.eb[lane] = 0u;
+    auto below = [&](int v) -> int {  // referenced rows before row v
+        if (v < 32) return __popc(m0 & ((1u << v) - 1u));
+        if (v < 64) return __popc(m0) + __popc(m1 & ((1u << (v - 32)) - 1u));
+        return __popc(m0) + __popc(m1) + __popc(m2 & ((1u << (v - 64)) - 1u));
+    };
+    auto used = [&](int v) -> bool {
+        const unsigned m = v < 32 ? m0 : v < 64 ? m1 : m2;
+        return (m >> (v & 31)) & 1u;
+    };
+    double *const cols[9] = {L.wx, L.wy, L.wz, L.ax, L.ay, L.az, L.bx, L.by, L.bz};
+    const int v1 = kWave + lane;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        // rows only move down: every lane reads before any lane writes
+        const double x0 = lane < nv ? cols[c][lane] : 0.0;
+        const double x1 = v1 < nv ? cols[c][v1] : 0.0;
+        __syncthreads();
+        if (lane < nv && used(lane)) cols[c][below(lane)] = x0;
+        if (v1 < nv && used(v1)) cols[c][below(v1)] = x1;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl) {
+        const int k = sl * kWave + lane;
+        if (k < nf) {
+            L.fv0[k] = below(L.fv0[k]);
+            L.fv1[k] = below(L.fv1[k]);
+            L.fv2[k] = below(L.fv2[k]);
+        }
+    }
+    __syncthreads();
+    return __popc(m0) + __popc(m1) + __popc(m2);
+}
+
 __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
     d3d_scene s, const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, int64_t n,
     const int64_t *__restrict__ n_dev, const int32_t *__restrict__ work,
     const int32_t *__restrict__ work_off, const double *__restrict__ simplex,
     const int32_t *__restrict__ npts, int max_iter, int max_faces, double eps,
     double *__restrict__ out, int32_t *__restrict__ status_out, int32_t *__restrict__ iters_out,
-    unsigned long long *__restrict__ next) {
+    unsigned long long *__restrict__ next, double *__restrict__ faces_out,
+    int32_t *__restrict__ nfaces_out) {
     __shared__ EpaLds L;
     const int lane = threadIdx.x;
     const int64_t count = n_dev ? *n_dev : n;
@@ -321,6 +382,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             if (lane == 0) {
                 status_out[pid] = 0x200;
                 if (iters_out) iters_out[pid] = 0;
+                if (nfaces_out) nfaces_out[pid] = 0;
             }
             continue;
         }
@@ -402,6 +464,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             if (lane == 0) {
                 status_out[pid] = 3;
                 if (iters_out) iters_out[pid] = 0;
+                if (nfaces_out) nfaces_out[pid] = 0;
             }
             __syncthreads();
             continue;
@@ -439,8 +502,11 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                 break;
             }
             if (nv >= kEpaMaxVerts) {
-                status = 2;
-                break;
+                nv = epa_compact_rows(L, nv, nf, lane);
+                if (nv >= kEpaMaxVerts) {
+                    status = 2;
+                    break;
+                }
             }
             const int wi = nv;
             if (lane == 0) {
@@ -624,6 +690,25 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             o[10] = cb.x; o[11] = cb.y; o[12] = cb.z;
             status_out[pid] = status;
             if (iters_out) iters_out[pid] = iterations;
+            if (nfaces_out) nfaces_out[pid] = nf;
+        }
+        // the final polytope, Polytope.get_all_faces (epa.py:70,78,130-131):
+        // per face its three vertices and its unit normal, (max_faces, 4, 3)
+        // per pair; rows past nf are left as they are
+        if (faces_out) {
+            double *fo = faces_out + (int64_t)12 * max_faces * pid;
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int k = sl * kWave + lane;
+                if (k < nf) {
+                    const v3 w0 = lds_w(L, L.fv0[k]), w1 = lds_w(L, L.fv1[k]), w2 = lds_w(L, L.fv2[k]);
+                    double *f = fo + 12 * k;
+                    f[0] = w0.x; f[1] = w0.y; f[2] = w0.z;
+                    f[3] = w1.x; f[4] = w1.y; f[5] = w1.z;
+                    f[6] = w2.x; f[7] = w2.y; f[8] = w2.z;
+                    f[9] = L.fnx[k]; f[10] = L.fny[k]; f[11] = L.fnz[k];
+                }
+            }
         }
         __syncthreads();
     }
@@ -633,7 +718,7 @@ int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t
                const int64_t *n_dev, const int32_t *work, const int32_t *work_off,
                const double *simplex, const int32_t *npts, int max_iter, int max_faces,
                double eps, double *out, int32_t *status, int32_t *iters,
-               unsigned long long *next, hipStream_t stream) {
+               unsigned long long *next, hipStream_t stream, double *faces, int32_t *n_faces) {
     if (n_max <= 0) return D3D_OK;
     int64_t grid = n_max;
     if (n_dev) {
@@ -647,7 +732,7 @@ int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t
     }
     hipLaunchKernelGGL(epa_kernel, dim3((unsigned)grid), dim3(kEpaBlock), 0, stream, s, pa, pb, n_max,
                        n_dev, work, work_off, simplex, npts, max_iter, max_faces, eps, out, status,
-                       iters, next);
+                       iters, next, faces, n_faces);
     D3D_CHECK_LAUNCH("epa_kernel");
     return D3D_OK;
 }
@@ -659,16 +744,16 @@ using namespace d3d;
 extern "C" int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
                              int64_t n, const double *simplex, const int32_t *n_points,
                              int32_t max_iterations, int32_t max_faces, double epsilon, double *out,
-                             int32_t *status, int32_t *iterations, void *stream) {
+                             int32_t *status, int32_t *iterations, double *faces,
+                             int32_t *n_faces, void *stream) {
     clear_error();
     if (int rc = check_scene(scene)) return rc;
     D3D_CHECK_ARG(n >= 0, "n must be >= 0");
     if (n == 0) return D3D_OK;
     D3D_CHECK_ARG(pair_a && pair_b && simplex && n_points && out && status, "NULL buffer");
-    D3D_CHECK_ARG(max_iterations >= 1 && max_iterations <= kEpaMaxIter,
-                  "max_iterations must be in [1, 64]");
+    D3D_CHECK_ARG(max_iterations >= 1, "max_iterations must be >= 1");
     D3D_CHECK_ARG(max_faces >= 4 && max_faces <= kEpaMaxFaces, "max_faces must be in [4, 128]");
     return launch_epa(*scene, pair_a, pair_b, n, nullptr, nullptr, nullptr, simplex, n_points,
                       (int)max_iterations, (int)max_faces, epsilon, out, status, iterations,
-                      nullptr, (hipStream_t)stream);
+                      nullptr, (hipStream_t)stream, faces, n_faces);
 }

@@ -818,8 +818,7 @@ extern "C" int d3d_gjk_epa_batch_ws(const d3d_scene *scene, const int32_t *pair_
     if (n == 0) return D3D_OK;
     D3D_CHECK_ARG(workspace && epa_out && epa_status, "NULL buffer");
     D3D_CHECK_ARG(workspace_bytes >= d3d_gjk_epa_workspace_bytes(n), "workspace too small");
-    D3D_CHECK_ARG(epa_max_iterations >= 1 && epa_max_iterations <= 64,
-                  "epa_max_iterations must be in [1, 64]");
+    D3D_CHECK_ARG(epa_max_iterations >= 1, "epa_max_iterations must be >= 1");
     D3D_CHECK_ARG(epa_max_faces >= 4 && epa_max_faces <= 128, "epa_max_faces must be in [4, 128]");
     char *w = (char *)workspace;
     const int64_t gq = a256(d3d_gjk_workspace_bytes(n));

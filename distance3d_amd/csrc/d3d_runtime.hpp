@@ -71,11 +71,13 @@ inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 
 // EPA over the pairs work[*work_off + i], i < *n_dev (device values; work_off
 // may be NULL), or over pairs 0..n_max-1 when work is NULL.  With `next` (a
 // device counter, zero at launch) wavefronts claim items from it one at a
-// time; without it each takes a fixed stride of items.
+// time; without it each takes a fixed stride of items.  faces / n_faces
+// (optional) receive the final polytope of each pair.
 int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n_max,
                const int64_t *n_dev, const int32_t *work, const int32_t *work_off,
                const double *simplex, const int32_t *npts, int max_iter, int max_faces,
                double eps, double *out, int32_t *status, int32_t *iters,
-               unsigned long long *next, hipStream_t stream);
+               unsigned long long *next, hipStream_t stream, double *faces = nullptr,
+               int32_t *n_faces = nullptr);
 
 }  // namespace d3d

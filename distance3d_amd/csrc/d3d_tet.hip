@@ -578,7 +578,9 @@ extern "C" int d3d_polygon_angles(const double *y, const double *x, int64_t n, d
 extern "C" int d3d_intersect_halfplanes(const double *halfplanes, int32_t n, double *points,
                                         int32_t *count, void *stream) {
     clear_error();
-    D3D_CHECK_ARG(n >= 0 && n <= (1 << 20), "n must be in [0, 2^20]");
+    // one wavefront does O(n^3) work (every pair against every half-plane);
+    // the reference calls it with at most 8 half-planes
+    D3D_CHECK_ARG(n >= 0 && n <= D3D_MAX_HALFPLANES, "n must be in [0, D3D_MAX_HALFPLANES = 1024]");
     D3D_CHECK_ARG(count && (n == 0 || (halfplanes && points)), "NULL buffer");
     hipLaunchKernelGGL(halfplane_kernel, dim3(1), dim3(kWave), 0, (hipStream_t)stream, halfplanes,
                        n, points, count);

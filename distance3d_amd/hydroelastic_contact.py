@@ -106,13 +106,17 @@ def intersect_tetrahedron_pairs(pairs, tetrahedra_points1, tetrahedra_points2, e
     pr = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
     if len(pr) == 0:
         return False, [], [], [], []
-    # the reference looks X up per pair (X1[i], X2[j]): a missing entry of a
-    # dict is a KeyError there too
-    for X, col in ((X1, 0), (X2, 1)):
-        if isinstance(X, dict):
-            missing = set(np.unique(pr[:, col]).tolist()) - set(X)
-            if missing:
-                raise KeyError(min(missing))
+    # the reference looks X up pair by pair, X1[i] then X2[j]: the first
+    # missing entry of a dict in that order is the KeyError it raises
+    d1, d2 = isinstance(X1, dict), isinstance(X2, dict)
+    if d1 or d2:
+        k1 = set(X1) if d1 else None
+        k2 = set(X2) if d2 else None
+        for i, j in pr.tolist():
+            if d1 and i not in k1:
+                raise KeyError(i)
+            if d2 and j not in k2:
+                raise KeyError(j)
     hp, planes, nv, polys, _ = _intersecting(T1, epsilon1, _dense(X1, len(T1)), T2, epsilon2,
                                              _dense(X2, len(T2)), pr, youngs_modulus1,
                                              youngs_modulus2, False)
@@ -120,13 +124,20 @@ def intersect_tetrahedron_pairs(pairs, tetrahedra_points1, tetrahedra_points2, e
             hp[:, 0].tolist(), hp[:, 1].tolist())
 
 
+MAX_HALFPLANES = 1024  # D3D_MAX_HALFPLANES
+
+
 def intersect_halfplanes(halfplanes):
     """Polygon points of a set of half-planes, rows (p, pq)
     (_halfplanes.py:29-64), on the GPU (d3d_intersect_halfplanes); raises
     AssertionError where the reference does."""
     import torch
-    hp = torch.as_tensor(np.ascontiguousarray(halfplanes, dtype=np.float64).reshape(-1, 4)).cuda()
+    hp = np.ascontiguousarray(halfplanes, dtype=np.float64).reshape(-1, 4)
     n = hp.shape[0]
+    if n > MAX_HALFPLANES:
+        raise ValueError("at most %d half-planes (one wavefront does O(n^3) work), got %d"
+                         % (MAX_HALFPLANES, n))
+    hp = torch.as_tensor(hp).cuda()
     pts = torch.empty((max(3 * n, 1), 2), dtype=torch.float64, device="cuda")
     count = torch.empty(1, dtype=torch.int32, device="cuda")
     kernels.check(kernels.lib().d3d_intersect_halfplanes(kernels.ptr(hp), n, kernels.ptr(pts),

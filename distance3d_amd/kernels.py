@@ -264,9 +264,12 @@ def tet_contact_forces(T1, e1, pairs, status, planes, n_vertices, polygons, youn
 
 
 class EpaResult:
-    """Outputs of :func:`epa_batch` (device tensors)."""
+    """Outputs of :func:`epa_batch` (device tensors); ``faces`` (n,
+    max_faces, 4, 3) and ``n_faces`` (n) only with ``want_faces``."""
 
-    def __init__(self, out, status, iterations):
+    def __init__(self, out, status, iterations, faces=None, n_faces=None):
+        self.faces = faces
+        self.n_faces = n_faces
         self.raw = out
         self.depth = out[:, 0]
         self.normal = out[:, 1:4]
@@ -282,9 +285,10 @@ class EpaResult:
 
 
 def epa_batch(scene, pair_a, pair_b, simplex, n_points, max_iterations=64, max_faces=128,
-              epsilon=1e-8, stream=None):
+              epsilon=1e-8, stream=None, want_faces=False):
     """EPA over pairs seeded from GJK simplices ((n,3,4,3) Y/P/Q rows and
-    n_points from :func:`gjk_distance_batch` with ``want_simplex=True``)."""
+    n_points from :func:`gjk_distance_batch` with ``want_simplex=True``).
+    With ``want_faces`` the final polytope of every pair is returned too."""
     import torch
     a = _dev_i32(pair_a, scene.device)
     b = _dev_i32(pair_b, scene.device)
@@ -296,10 +300,16 @@ def epa_batch(scene, pair_a, pair_b, simplex, n_points, max_iterations=64, max_f
     out = torch.empty((n, 13), dtype=torch.float64, device=scene.device)
     status = torch.empty(n, dtype=torch.int32, device=scene.device)
     iters = torch.empty(n, dtype=torch.int32, device=scene.device)
+    faces = n_faces = None
+    if want_faces:
+        faces = torch.zeros((n, max(int(max_faces), 0), 4, 3), dtype=torch.float64,
+                            device=scene.device)
+        n_faces = torch.zeros(n, dtype=torch.int32, device=scene.device)
     check(lib().d3d_epa_batch(scene.c, ptr(a), ptr(b), n, ptr(s), ptr(npts),
                               int(max_iterations), int(max_faces), float(epsilon), ptr(out),
-                              ptr(status), ptr(iters), stream_ptr(stream)), "d3d_epa_batch")
-    return EpaResult(out, status, iters)
+                              ptr(status), ptr(iters), ptr(faces), ptr(n_faces),
+                              stream_ptr(stream)), "d3d_epa_batch")
+    return EpaResult(out, status, iters, faces, n_faces)
 
 
 class Bvh:

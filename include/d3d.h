@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define D3D_ABI_VERSION 2
+#define D3D_ABI_VERSION 3
 
 /* error codes */
 #define D3D_OK 0
@@ -221,18 +221,25 @@ int d3d_gjk_intersection_batch(const d3d_scene *scene, const int32_t *pair_a,
  * simplices of d3d_gjk_distance_batch (simplex (n,3,4,3) f64 = Y, P, Q rows,
  * n_points (n) int32) of intersecting pairs.  One wavefront per pair, polytope
  * in LDS.
- *   max_iterations in [1, 64] (epa.py's max_iter), max_faces in [4, 128],
+ *   max_iterations >= 1 (epa.py's max_iter), max_faces in [4, 128],
  *   epsilon: convergence tolerance (1e-8).
  *   out (n,13) f64: depth, normal[3], mtv[3] = normal * depth, contact point
  *       on A [3], contact point on B [3] (cp_a - cp_b = mtv when converged).
- *   status (n) int32: 1 converged, 0 max_iterations reached, 2 face cap,
+ *   status (n) int32: 1 converged, 0 max_iterations reached, 2 face cap (or
+ *       the polytope's 72 vertex rows used up: 128 faces need about 66),
  *       3 degenerate seed (flat Minkowski difference), 0x200 bad input.
  *   iterations (n) int32, optional.
+ *   faces (n, max_faces, 4, 3) f64, optional: the final polytope,
+ *       Polytope.get_all_faces (epa.py:70,78): per face its three vertices
+ *       (Minkowski points) and its outward unit normal; n_faces (n) int32,
+ *       optional: the number of faces written per pair (rows past it are
+ *       not touched).
  */
 int d3d_epa_batch(const d3d_scene *scene, const int32_t *pair_a, const int32_t *pair_b,
                   int64_t n, const double *simplex, const int32_t *n_points,
                   int32_t max_iterations, int32_t max_faces, double epsilon, double *out,
-                  int32_t *status, int32_t *iterations, void *stream);
+                  int32_t *status, int32_t *iterations, double *faces, int32_t *n_faces,
+                  void *stream);
 
 /*
  * GJK then EPA in one device-side pipeline (gjk.gjk followed by epa.epa on
@@ -324,8 +331,11 @@ int d3d_polygon_angles(const double *y, const double *x, int64_t n, double *out,
  * pairwise intersections of n half-planes (rows p.x, p.y, pq.x, pq.y, f64)
  * that lie inside all the others, in the reference's (i, j > i) order, to
  * points (3n, 2); *count (device int32) = their number, or -1 where the
- * reference asserts (more points than it reserves).  One wavefront.
+ * reference asserts (more points than it reserves).  One wavefront does
+ * O(n^3) work, so n is limited to D3D_MAX_HALFPLANES (the reference calls it
+ * with at most 8 half-planes per tetrahedron pair).
  */
+#define D3D_MAX_HALFPLANES 1024
 int d3d_intersect_halfplanes(const double *halfplanes, int32_t n, double *points, int32_t *count,
                              void *stream);
 

@@ -662,13 +662,33 @@ static int epa_seed(const Scene *s, int ka, int kb, const V *Y, const V *P, cons
     return 1;
 }
 
+/* Drops vertex rows no face refers to and renumbers the rest in order (the
+ * faces follow).  Vertices are only referred to by index, so the expansion is
+ * unchanged; this bounds the rows a long expansion needs.  Returns the new
+ * row count. */
+static int epa_compact_rows(V *W, V *PA, V *PB, int nv, Face *F, int nf) {
+    int used[EPA_MAX_VERTS], to[EPA_MAX_VERTS];
+    for (int v = 0; v < nv; ++v) used[v] = 0;
+    for (int k = 0; k < nf; ++k)
+        for (int e = 0; e < 3; ++e) used[F[k].v[e]] = 1;
+    int m = 0;
+    for (int v = 0; v < nv; ++v) {
+        to[v] = m;
+        if (used[v]) { W[m] = W[v]; PA[m] = PA[v]; PB[m] = PB[v]; ++m; }
+    }
+    for (int k = 0; k < nf; ++k)
+        for (int e = 0; e < 3; ++e) F[k].v[e] = to[F[k].v[e]];
+    return m;
+}
+
+/* faces_out (optional): the final polytope, max_faces rows of (v0, v1, v2,
+ * unit normal), Polytope.get_all_faces (epa.py:130-131). */
 static void epa_pair(const Scene *s, int ka, int kb, const V *Y, const V *P, const V *Q, int n,
-                     int max_iter, int max_faces, double eps, EpaOut *o) {
+                     int max_iter, int max_faces, double eps, EpaOut *o, double *faces_out) {
     V W[EPA_MAX_VERTS], PA[EPA_MAX_VERTS], PB[EPA_MAX_VERTS];
     Face F[EPA_MAX_FACES_CAP], NF[EPA_MAX_FACES_CAP];
     int nv = 0;
     if (max_faces > EPA_MAX_FACES_CAP) max_faces = EPA_MAX_FACES_CAP;
-    if (max_iter > EPA_MAX_VERTS - 8) max_iter = EPA_MAX_VERTS - 8;
     memset(o, 0, sizeof(*o));
     if (!epa_seed(s, ka, kb, Y, P, Q, n, W, PA, PB, &nv)) { o->status = 3; return; }
     /* orient the seed: vertex 3 must be behind face (0,1,2) */
@@ -702,7 +722,10 @@ static void epa_pair(const Scene *s, int ka, int kb, const V *Y, const V *P, con
             o->mtv = scl(cf.n, wn);
             break;
         }
-        if (nv >= EPA_MAX_VERTS) { status = 2; break; }
+        if (nv >= EPA_MAX_VERTS) {
+            nv = epa_compact_rows(W, PA, PB, nv, F, nf);
+            if (nv >= EPA_MAX_VERTS) { status = 2; break; }
+        }
         int wi = nv;
         W[nv] = w; PA[nv] = pa; PB[nv] = pb; ++nv;
         /* faces that see w */
@@ -754,6 +777,14 @@ static void epa_pair(const Scene *s, int ka, int kb, const V *Y, const V *P, con
     o->status = status;
     o->iterations = it + (status == 1 ? 1 : 0);
     o->n_faces = nf;
+    if (faces_out)
+        for (int k = 0; k < nf; ++k) {
+            double *fo = faces_out + 12 * k;
+            for (int e = 0; e < 3; ++e) {
+                fo[3 * e] = W[F[k].v[e]].x; fo[3 * e + 1] = W[F[k].v[e]].y; fo[3 * e + 2] = W[F[k].v[e]].z;
+            }
+            fo[9] = F[k].n.x; fo[10] = F[k].n.y; fo[11] = F[k].n.z;
+        }
     if (nf > 0) {
         /* contact points: barycentric coordinates of depth * n in the chosen
          * face, applied to the support points on A and B */
@@ -1537,7 +1568,8 @@ void orc_gjk_intersection(const double *f, const int32_t *i, const float *vx, co
 void orc_epa(const double *f, const int32_t *i, const float *vx, const float *vy,
              const float *vz, const double *v64, const int32_t *pa, const int32_t *pb,
              int64_t n, const double *simplex, const int32_t *n_points, int max_iter,
-             int max_faces, double eps, double *out, int32_t *status, int32_t *iters) {
+             int max_faces, double eps, double *out, int32_t *status, int32_t *iters,
+             double *faces, int32_t *n_faces) {
     Scene s = mkscene(f, i, vx, vy, vz, v64);
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t k = 0; k < n; ++k) {
@@ -1549,7 +1581,9 @@ void orc_epa(const double *f, const int32_t *i, const float *vx, const float *vy
             Q[r] = mk(S[24 + 3 * r], S[25 + 3 * r], S[26 + 3 * r]);
         }
         EpaOut o;
-        epa_pair(&s, pa[k], pb[k], Y, P, Q, n_points[k], max_iter, max_faces, eps, &o);
+        epa_pair(&s, pa[k], pb[k], Y, P, Q, n_points[k], max_iter, max_faces, eps, &o,
+                 faces ? faces + 12 * (int64_t)max_faces * k : NULL);
+        if (n_faces) n_faces[k] = o.n_faces;
         double *O = out + 13 * k;
         O[0] = o.depth;
         O[1] = o.normal.x; O[2] = o.normal.y; O[3] = o.normal.z;

@@ -103,7 +103,9 @@ def gjk_intersection(scene, pair_a, pair_b, tolerance=1e-10, max_iterations=128)
 
 
 def epa(scene, pair_a, pair_b, simplex, n_points, max_iterations=64, max_faces=128,
-        epsilon=1e-8):
+        epsilon=1e-8, want_faces=False):
+    """EPA per pair; with ``want_faces`` also "faces" (n, max_faces, 4, 3),
+    the final polytope (rows past "n_faces" are zero)."""
     a = np.ascontiguousarray(pair_a, dtype=np.int32)
     b = np.ascontiguousarray(pair_b, dtype=np.int32)
     n = len(a)
@@ -112,12 +114,18 @@ def epa(scene, pair_a, pair_b, simplex, n_points, max_iterations=64, max_faces=1
     out = np.empty((n, 13))
     status = np.empty(n, dtype=np.int32)
     iters = np.empty(n, dtype=np.int32)
+    faces = np.zeros((n, max_faces, 4, 3)) if want_faces else None
+    n_faces = np.zeros(n, dtype=np.int32)
     lib().orc_epa(*_scene_args(scene), _p(a), _p(b), ctypes.c_int64(n), _p(s), _p(npts),
                   ctypes.c_int(max_iterations), ctypes.c_int(max_faces),
-                  ctypes.c_double(epsilon), _p(out), _p(status), _p(iters))
-    return {"depth": out[:, 0], "normal": out[:, 1:4], "mtv": out[:, 4:7],
-            "cp_a": out[:, 7:10], "cp_b": out[:, 10:13], "status": status,
-            "iterations": iters}
+                  ctypes.c_double(epsilon), _p(out), _p(status), _p(iters), _p(faces),
+                  _p(n_faces))
+    r = {"depth": out[:, 0], "normal": out[:, 1:4], "mtv": out[:, 4:7],
+         "cp_a": out[:, 7:10], "cp_b": out[:, 10:13], "status": status,
+         "iterations": iters, "n_faces": n_faces}
+    if want_faces:
+        r["faces"] = faces
+    return r
 
 
 def mpr(scene, pair_a, pair_b, centers, tolerance=0.0001, max_iterations=100, penetration=True):

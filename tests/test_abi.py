@@ -64,3 +64,20 @@ def test_argument_errors_without_gpu():
     assert lib.d3d_tet_pairs_workspace_bytes(0) == 0
     assert lib.d3d_tet_pairs_workspace_bytes(100) == 128 * 48 * 8
     assert lib.d3d_tet_pairs_workspace_bytes(1 << 30) == 4096 * 64 * 48 * 8
+    # intersect_halfplanes: one wavefront does O(n^3) work; n is capped
+    assert lib.d3d_intersect_halfplanes(None, 1025, None, None, None) == invalid
+    assert b"1024" in lib.d3d_last_error()
+    assert lib.d3d_intersect_halfplanes(None, -1, None, None, None) == invalid
+    # EPA: max_iterations below 1 and max_faces outside [4, 128] are rejected
+    # before any pointer is used (the pointers below are never dereferenced)
+    import ctypes
+    sc = _native.d3d_scene()
+    sc.shape_f64 = sc.shape_i32 = 8
+    sc.n_shapes = 1
+    fake = ctypes.c_void_p(8)
+    args = [ctypes.byref(sc), fake, fake, 1, fake, fake]
+    assert lib.d3d_epa_batch(*args, 0, 64, 1e-8, fake, fake, None, None, None, None) == invalid
+    assert b"max_iterations" in lib.d3d_last_error()
+    assert lib.d3d_epa_batch(*args, 64, 129, 1e-8, fake, fake, None, None, None, None) == invalid
+    assert b"max_faces" in lib.d3d_last_error()
+    assert lib.d3d_epa_batch(*args, 64, 3, 1e-8, fake, fake, None, None, None, None) == invalid

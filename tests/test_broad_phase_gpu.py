@@ -279,3 +279,25 @@ def test_self_collision_detect(gpu):  # self_collision.py:5-64
             expect[frames[i]] = expect[frames[j]] = True
     assert contacts == expect
     assert self_collision.detect_any(bvh) == any(expect.values())
+
+
+def test_cube_sphere_unique_pairs_edge_case(gpu):  # test_cube_unique_pairs_edge_case.py:8-31
+    """Exactly 162 tetrahedron-box pairs, through both the brute-force
+    overlap kernel (all_aabbs_overlap) and the tree-vs-tree query
+    (AabbTree.overlaps_aabb_tree), equal to the oracle's set."""
+    from tests.test_oracle_pins import cube_sphere_tetrahedron_boxes
+    rb1, rb2 = cube_sphere_tetrahedron_boxes()
+    # tetrahedron boxes computed on the GPU, bit-equal to the oracle's
+    a1, a2 = rb1.aabbs, rb2.aabbs
+    assert np.array_equal(a1, oracle.tet_mesh_prep(rb1.tetrahedra_points)[0])
+    assert np.array_equal(a2, oracle.tet_mesh_prep(rb2.tetrahedra_points)[0])
+    want, cnt = oracle.all_aabbs_overlap(a1, a2)
+    assert cnt == 162
+    _, _, dense = aabb_tree.all_aabbs_overlap(a1, a2)
+    assert len(dense) == 162
+    assert [tuple(p) for p in want.tolist()] == dense  # row-major order
+    _, u1, u2, tree = rb1.aabb_tree.overlaps_aabb_tree(rb2.aabb_tree)
+    assert len(tree) == 162
+    assert set(tree) == set(dense)
+    assert np.array_equal(u1, np.unique(want[:, 0]))
+    assert np.array_equal(u2, np.unique(want[:, 1]))

@@ -261,6 +261,18 @@ def test_missing_transform_is_a_key_error():
         hc.intersect_tetrahedron_pairs([(0, 0), (1, 2)], T, T, e, e, X, X)
     with pytest.raises(KeyError):
         hc.intersect_tetrahedron_pairs([(0, 0), (2, 1)], T, T, e, e, X, X)
+    # the first missing key in pair order, X1[i] before X2[j]: pairs (0, 5),
+    # (3, 0) with X1 missing 3 and X2 missing 5 raise for 5
+    T6 = random_tets(np.random.RandomState(18), 6)
+    e6 = np.ones((6, 4))
+    X1 = {k: np.eye(4) for k in (0, 1, 2, 4, 5)}
+    X2 = {k: np.eye(4) for k in (0, 1, 2, 3, 4)}
+    with pytest.raises(KeyError) as info:
+        hc.intersect_tetrahedron_pairs([(0, 5), (3, 0)], T6, T6, e6, e6, X1, X2)
+    assert info.value.args == (5,)
+    with pytest.raises(KeyError) as info:
+        hc.intersect_tetrahedron_pairs([(3, 5)], T6, T6, e6, e6, X1, X2)
+    assert info.value.args == (3,)
 
 
 def test_rigid_body_transforms():  # test_hydroelastic_contact.py:67-86

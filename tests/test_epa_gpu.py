@@ -150,3 +150,120 @@ def test_dropin_epa_separation(gpu):  # test_epa.py:37-60
         assert gjk.gjk(c1, moved)[0] == approx(1.0)
         done += 1
     assert done > 10
+
+
+def test_epa_contact_points_on_both_hulls(gpu):
+    """GPU contact points lie on the boundary of their own hull (largest
+    Qhull facet offset zero) and reproduce the MTV, on random hull pairs."""
+    from scipy.spatial import ConvexHull
+    rs = np.random.RandomState(26)
+    b = SceneBuilder()
+    verts = []
+    for _ in range(120):
+        v1 = rs.randn(rs.randint(5, 14), 3)
+        v2 = rs.randn(rs.randint(5, 14), 3) * 0.8 + rs.randn(3) * 0.5
+        b.add_hull(v1)
+        b.add_hull(v2)
+        verts += [v1, v2]
+    host = b.build()
+    pa = np.arange(0, 240, 2, dtype=np.int32)
+    pb = pa + 1
+    r, o, idx = _epa_both(host, pa, pb)
+    st = _np(r.status)
+    done = 0
+    for k, i in enumerate(idx):
+        if st[k] != 1:
+            continue
+        for v, cp in ((verts[2 * i], _np(r.cp_a)[k]), (verts[2 * i + 1], _np(r.cp_b)[k])):
+            eq = ConvexHull(v).equations
+            assert abs((eq[:, :3] @ cp + eq[:, 3]).max()) < 1e-9
+        assert_array_almost_equal(_np(r.cp_a)[k] - _np(r.cp_b)[k], _np(r.mtv)[k], decimal=12)
+        done += 1
+    assert done > 40
+
+
+def test_epa_faces_match_oracle(gpu):
+    """The final polytope (faces output) equals the oracle's, face for face
+    and bit for bit, in the same order."""
+    w = workloads.epa_pairs(seed=12, n_pairs=600)
+    dev = w.scene.to_device()
+    g = kernels.gjk_distance_batch(dev, w.pair_a, w.pair_b, want_simplex=True)
+    idx = np.nonzero(_np(g.dist) == 0.0)[0]
+    pa, pb = np.asarray(w.pair_a)[idx], np.asarray(w.pair_b)[idx]
+    simplex, npts = _np(g.simplex)[idx], _np(g.n_points)[idx]
+    r = kernels.epa_batch(dev, pa, pb, simplex, npts, want_faces=True)
+    o = oracle.epa(w.scene, pa, pb, simplex, npts, want_faces=True)
+    assert np.array_equal(_np(r.status), o["status"])
+    nf = _np(r.n_faces)
+    assert np.array_equal(nf, o["n_faces"])
+    F = _np(r.faces)
+    for k in range(len(idx)):
+        assert np.array_equal(F[k, :nf[k]], o["faces"][k, :nf[k]])
+
+
+def test_epa_dropin_faces_and_caller_simplex(gpu):  # test_epa.py:7-34
+    """epa.epa returns the polytope (n_faces, 4, 3) with unit normals whose
+    closest face lies within epsilon of the depth; a plain (4, 3) simplex
+    whose rows span a tetrahedron seeds the polytope as given, with the
+    oracle's result from the same rows."""
+    from tests.test_oracle_pins import _closed_convex_polytope
+    v1 = np.array([[1.76405235, 0.40015721, 0.97873798], [2.2408932, 1.86755799, -0.97727788],
+                   [0.4105985, 0.14404357, 1.45427351], [0.33367433, 1.49407907, -0.20515826],
+                   [0.3130677, -0.85409574, -2.55298982], [2.26975462, -1.45436567, 0.04575852],
+                   [-0.18718385, 1.53277921, 1.46935877]])
+    v2 = np.array([[-2.32605299, -0.31242692, 0.07599278], [0.88503416, 1.23786508, -0.467683],
+                   [-0.64755927, -1.01306774, -1.50037412], [-2.05152671, 1.98626062, -0.59000837],
+                   [-0.78333082, -1.21731013, 0.69713417], [-1.95915437, -0.17725505, -0.97582275],
+                   [0.04164598, -0.47531991, -1.26098837], [-0.37343875, 0.4638171, -0.01383896],
+                   [-0.04278462, -0.59883687, -0.44309735]])
+    c1, c2 = colliders.ConvexHullVertices(v1), colliders.ConvexHullVertices(v2)
+    dist, p1, p2, simplex = gjk.gjk(c1, c2)
+    assert dist == 0.0
+    mtv, faces, success = epa.epa(simplex, c1, c2)
+    assert success
+    assert_array_almost_equal(mtv, [-0.387287, 0.179576, -0.176204])
+    assert faces.ndim == 3 and faces.shape[1:] == (4, 3) and len(faces) >= 4
+    md = (v1[:, None, :] - v2[None, :, :]).reshape(-1, 3)
+    _closed_convex_polytope(faces, np.linalg.norm(mtv), md)
+    # a caller-built simplex: four Minkowski-difference points spanning a
+    # tetrahedron around the origin, passed as a plain array
+    b = SceneBuilder()
+    b.add_hull(v1)
+    b.add_hull(v2)
+    host = b.build()
+    Y = np.asarray(simplex, dtype=np.float64)[:4] * 0.5
+    if simplex.n_points < 4:
+        Y = 1e-3 * np.array([[1.0, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]])
+    mtv2, faces2, ok2 = epa.epa(np.array(Y), c1, c2)
+    seed = np.stack([Y, Y, np.zeros((4, 3))])[None]
+    o = oracle.epa(host, [0], [1], seed, [4], max_faces=64, want_faces=True)
+    assert ok2 == (o["status"][0] == 1)
+    assert np.array_equal(mtv2, o["mtv"][0])
+    assert np.array_equal(faces2, o["faces"][0, :o["n_faces"][0]])
+    assert_array_almost_equal(mtv2, [-0.387287, 0.179576, -0.176204])
+    # max_iter past 64 is accepted and converges to the same answer
+    mtv3, _, ok3 = epa.epa(simplex, c1, c2, max_iter=500)
+    assert ok3 and np.array_equal(mtv3, mtv)
+
+
+def test_epa_long_stalled_run_matches_oracle(gpu):
+    """epsilon = 0 and 400 iterations from a tiny caller-built seed: the
+    kernel recycles its 72 vertex rows many times (the oracle at 260), with
+    the oracle's status, depth, normal, contact points and polytope."""
+    from tests.test_oracle_pins import stalled_epa_case
+    host, pa, pb, simplex, npts = stalled_epa_case()
+    dev = host.to_device()
+    r = kernels.epa_batch(dev, pa, pb, simplex, npts, max_iterations=400, epsilon=0.0,
+                          want_faces=True)
+    o = oracle.epa(host, pa, pb, simplex, npts, max_iterations=400, epsilon=0.0,
+                   want_faces=True)
+    assert np.array_equal(_np(r.status), o["status"])
+    assert np.array_equal(_np(r.iterations), o["iterations"])
+    assert np.sum(o["iterations"] == 400) >= 5
+    assert np.array_equal(_np(r.raw), np.concatenate(
+        [o["depth"][:, None], o["normal"], o["mtv"], o["cp_a"], o["cp_b"]], axis=1))
+    nf = _np(r.n_faces)
+    assert np.array_equal(nf, o["n_faces"])
+    F = _np(r.faces)
+    for k in range(len(pa)):
+        assert np.array_equal(F[k, :nf[k]], o["faces"][k, :nf[k]])
