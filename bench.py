@@ -58,6 +58,9 @@ def parse():
                    help="comma list of extra configs: hull,epa,broad,serving,mpr,tet,contact")
     p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--serving-requests", type=int, default=50000)
+    p.add_argument("--classes", action="store_true",
+                   help="only the per-work-class breakdown of the headline workload: each "
+                        "class's pairs timed as a GJK batch of their own")
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="before the workload, run one device-to-device copy of this many MiB "
                         "(a kernel of known traffic for scaling profiler byte counters)")
@@ -108,6 +111,32 @@ def timed_steps(step, steps, warmup):
     return wall, np.array([a.elapsed_time(b) for a, b in ev])
 
 
+def device_copy_bandwidth(mib=1024, reps=10):
+    """Measured HBM bandwidth of a plain device-to-device copy (torch's copy
+    kernel): bytes read + written per copy over its event time, median of
+    ``reps``."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        ms.append(a.elapsed_time(b))
+    del src, dst
+    torch.cuda.empty_cache()
+    t = float(np.median(ms))
+    return {"GBps": round(2.0 * (mib << 20) / (t / 1e3) / 1e9, 1), "MiB": mib, "ms": round(t, 4),
+            "note": "torch device-to-device copy, read + write bytes over the event time, "
+                    "median of %d" % reps}
+
+
 def rel_err_stats(gpu, ref):
     sep = (ref > 1e-9) & (ref < MAXF)
     rel = np.abs(gpu[sep] - ref[sep]) / ref[sep]
@@ -119,6 +148,64 @@ def rel_err_stats(gpu, ref):
         "flag_mismatches_outside_1e-9_band": int(np.sum(((gpu == 0.0) != (ref == 0.0)) & ~band)),
         "intersecting_frac": float(np.mean(ref == 0.0)),
     }
+
+
+def cp_err_stats(cpa, cpb, ref):
+    """Closest-point errors against the oracle over the pairs it does not
+    clip: per pair the larger max-norm error of the two points, absolute and
+    relative to max(1, |p|_inf) (gjk/_gjk_jolt.py:213-221 defines the points)."""
+    ok = ref["dist"] < MAXF
+    if not ok.any():
+        return {"max_cp_err": 0.0, "p99_cp_err": 0.0, "max_rel_cp_err": 0.0}
+    e = np.maximum(np.abs(cpa[ok] - ref["cp_a"][ok]).max(axis=1),
+                   np.abs(cpb[ok] - ref["cp_b"][ok]).max(axis=1))
+    scale = np.maximum(1.0, np.maximum(np.abs(ref["cp_a"][ok]).max(axis=1),
+                                       np.abs(ref["cp_b"][ok]).max(axis=1)))
+    return {"max_cp_err": float(e.max()), "p99_cp_err": float(np.percentile(e, 99)),
+            "max_rel_cp_err": float((e / scale).max())}
+
+
+EPA_STATUS = {0: "iteration_cap", 1: "converged", 2: "face_cap", 3: "degenerate_seed"}
+
+
+def epa_status_counts(st):
+    return {EPA_STATUS[k]: int(np.sum(st == k)) for k in EPA_STATUS}
+
+
+def epa_unconverged_report(w, dev, idx, epa_out, max_iterations=64):
+    """What happens on the EPA pairs that stop without converging: the same
+    pairs rerun with 16x the iteration cap (how much the capped depth moves),
+    and MPR's penetration depth (mpr.py:52-109) on them for comparison."""
+    import torch
+    from distance3d_amd import kernels
+    from distance3d_amd.scene import collider_centers
+    if len(idx) == 0:
+        return {"pairs": 0}
+    pa, pb = w.pair_a[idx], w.pair_b[idx]
+    g = kernels.gjk_distance_batch(dev, pa, pb, want_simplex=True)
+    long_ = kernels.epa_batch(dev, pa, pb, g.simplex, g.n_points,
+                              max_iterations=16 * max_iterations)
+    d_cap = epa_out[idx, 0]
+    d_long = long_.depth.cpu().numpy()
+    st_long = long_.status.cpu().numpy()
+    centers = collider_centers(w.scene, only=np.union1d(pa, pb))
+    mst, mout = kernels.mpr_batch(dev, pa, pb, centers)
+    torch.cuda.synchronize()
+    mst = mst.cpu().numpy()
+    d_mpr = mout.cpu().numpy()[:, 0]
+    hit = (mst & 1) != 0
+    conv = st_long == 1
+    rel = np.abs(d_cap[conv] - d_long[conv]) / np.maximum(d_long[conv], 1e-300)
+    ratio = d_cap[hit] / np.maximum(d_mpr[hit], 1e-300)
+    return {"pairs": int(len(idx)),
+            "rerun_max_iterations": 16 * max_iterations,
+            "rerun_status_counts": epa_status_counts(st_long),
+            "capped_vs_rerun_max_rel_depth_diff": float(rel.max(initial=0.0)),
+            "capped_vs_rerun_median_rel_depth_diff": float(np.median(rel)) if len(rel) else 0.0,
+            "mpr_intersecting": int(hit.sum()),
+            "capped_depth_over_mpr_depth_median": float(np.median(ratio)) if len(ratio) else None,
+            "capped_depth_over_mpr_depth_min": float(ratio.min()) if len(ratio) else None,
+            "capped_depth_over_mpr_depth_max": float(ratio.max()) if len(ratio) else None}
 
 
 def epa_parity(gpu_out, gpu_status, ref):
@@ -136,6 +223,7 @@ def epa_parity(gpu_out, gpu_status, ref):
             np.abs(gpu_out[conv, 7:10] - ref["cp_a"][conv]).max(initial=0.0),
             np.abs(gpu_out[conv, 10:13] - ref["cp_b"][conv]).max(initial=0.0))),
         "bit_identical_depth_frac": float(np.mean(d == od)),
+        "status_counts": epa_status_counts(gpu_status),
     }
 
 
@@ -155,22 +243,41 @@ def run_pipeline_config(w, args, lanes, steps, warmup, label, parity=True, cpu=F
 
     wall, ev_ms = timed_steps(step, steps, warmup)
     value = n / (wall / 1e3)
+    # one more (untimed) pass for the per-pair GJK iteration counts
+    iters = torch.empty(n, dtype=torch.int32, device="cuda")
+    kernels.gjk_epa_batch(dev, pa, pb, out=out, workspace=ws, lanes_per_pair=lanes,
+                          iterations=iters)
+    iters = iters.cpu().numpy().astype(np.int64)
     dist = out.dist.cpu().numpy()
     hit = dist == 0.0
     # algorithmic bytes per pair: 12 (n_A + n_B) FP32 hull bytes, 2 x 64 pose
     # and parameters, 8 pair ids, 60 GJK outputs, +80 EPA outputs if hit
-    bytes_step = float(np.sum(w.hull_bytes_per_pair()) + n * (128 + 8 + 60) + 80 * hit.sum())
+    hull_bytes = w.hull_bytes_per_pair()
+    bytes_step = float(np.sum(hull_bytes) + n * (128 + 8 + 60) + 80 * hit.sum())
     achieved = bytes_step / (ev_ms.mean() / 1e3) / 1e9
+    # every GJK iteration evaluates both supports: a full sweep of both hulls
+    # would read 12 (n_A + n_B) bytes
+    sweep_bytes = float(np.sum(iters * hull_bytes))
     res = {"workload": label, "pairs": n, "value": round(value, 1), "unit": "pair-queries/s",
            "ms_per_step": round(wall, 4), "event_ms_per_step": round(float(ev_ms.mean()), 4),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                        "bytes_per_launch": bytes_step}}
+                        "bytes_per_launch": bytes_step},
+           "gjk_iterations": {"mean": round(float(iters.mean()), 3), "max": int(iters.max()),
+                              "total": int(iters.sum())},
+           "effective_sweep": {"bytes": sweep_bytes,
+                               "GBps_over_step": round(sweep_bytes / (ev_ms.mean() / 1e3) / 1e9, 1),
+                               "note": "sum over pairs of GJK iterations x 12 (n_A + n_B): the "
+                                       "bytes full support sweeps of both hulls at every "
+                                       "iteration would read, over the step time"}}
+    epa_st = out.epa_status.cpu().numpy()
+    res["epa_status_counts"] = epa_status_counts(epa_st)
     if parity:
         from oracle import oracle
         t0 = time.time()
         o = oracle.gjk_distance(w.scene, w.pair_a, w.pair_b, want_simplex=True)
         res["parity"] = rel_err_stats(dist, o["dist"])
+        res["parity"].update(cp_err_stats(out.cp_a.cpu().numpy(), out.cp_b.cpu().numpy(), o))
         idx = np.nonzero(o["dist"] == 0.0)[0]
         if len(idx):
             e = oracle.epa(w.scene, w.pair_a[idx], w.pair_b[idx], o["simplex"][idx],
@@ -178,6 +285,9 @@ def run_pipeline_config(w, args, lanes, steps, warmup, label, parity=True, cpu=F
             res["parity"]["epa"] = epa_parity(out.epa_out.cpu().numpy()[idx],
                                               out.epa_status.cpu().numpy()[idx], e)
         log("%s parity in %.1fs" % (label, time.time() - t0))
+    unconv = np.nonzero((epa_st == 0) | (epa_st == 2))[0]
+    if len(unconv):
+        res["epa_unconverged"] = epa_unconverged_report(w, dev, unconv, out.epa_out.cpu().numpy())
     if cpu:
         res["cpu_baseline"], res["cpu_single_core"] = cpu_baseline(w, args)
     return res
@@ -527,6 +637,53 @@ def run_contact(args, steps=10, warmup=2):
     return res
 
 
+def class_breakdown(w, lanes, reps=5):
+    """Each work class of the workload (pairs sorted by their total hull
+    vertex count, 8 classes) as a GJK batch of its own: event time, pairs,
+    GJK iterations, and the algorithmic bytes 12 (n_A + n_B) + 196 per pair
+    over that time.  Each class is one set of dispatches, so a kernel trace
+    of this mode gives per-class kernel times and counters."""
+    import torch
+    from distance3d_amd import autotune, kernels
+    dev = w.scene.to_device("cuda:0")
+    cls = autotune.pair_classes(w.scene, w.pair_a, w.pair_b)
+    hb = w.hull_bytes_per_pair()
+    rows = {}
+    for c in range(8):
+        idx = np.nonzero(cls == c)[0]
+        if len(idx) == 0:
+            continue
+        pa = torch.from_numpy(w.pair_a[idx]).cuda()
+        pb = torch.from_numpy(w.pair_b[idx]).cuda()
+        n = len(idx)
+        out = kernels.GjkResult(torch.empty(n, dtype=torch.float64, device="cuda"),
+                                torch.empty((n, 3), dtype=torch.float64, device="cuda"),
+                                torch.empty((n, 3), dtype=torch.float64, device="cuda"),
+                                torch.empty(n, dtype=torch.int32, device="cuda"),
+                                torch.empty(n, dtype=torch.int32, device="cuda"), None, None)
+        ws = torch.empty(kernels.gjk_workspace_bytes(n), dtype=torch.uint8, device="cuda")
+
+        def step():
+            kernels.gjk_distance_batch(dev, pa, pb, lanes_per_pair=lanes, out=out, workspace=ws)
+
+        log("class %d: %d pairs" % (c, n))
+        wall, ev = timed_steps(step, reps, 1)
+        it = out.iterations.cpu().numpy().astype(np.int64)
+        b = float(np.sum(hb[idx]) + 196 * n)
+        ms = float(ev.mean())
+        rows[str(c)] = {"pairs": n, "ms": round(ms, 4), "lanes": (lanes or list(
+            _native_default_lanes()))[c], "iterations_mean": round(float(it.mean()), 3),
+            "hull_vertices_mean": round(float(hb[idx].mean()) / 12.0, 1),
+            "algorithmic_GBps": round(b / (ms / 1e3) / 1e9, 1),
+            "sweep_GBps": round(float(np.sum(it * hb[idx])) / (ms / 1e3) / 1e9, 1)}
+    return rows
+
+
+def _native_default_lanes():
+    from distance3d_amd import _native
+    return _native.DEFAULT_LANES
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -547,10 +704,17 @@ def main():
         log("copy calibration: %d MiB read + %d MiB written" % (args.copy_calibration_mib,
                                                                args.copy_calibration_mib))
         del src, dst
+    copy_bw = device_copy_bandwidth()
+    log("device copy: %.1f GB/s" % copy_bw["GBps"])
     t0 = time.time()
     w = workloads.mixed_pairs(seed=args.seed, n_pairs=args.pairs)
     log("workload generated in %.1fs: %d pairs, %d hull vertices"
         % (time.time() - t0, w.n_pairs, w.scene.n_vertices))
+    if args.classes:
+        print(json.dumps({"class_breakdown": class_breakdown(w, lanes),
+                          "workload": "mixed_pairs seed %d, %d pairs" % (args.seed, w.n_pairs)}),
+              flush=True)
+        return
     head = run_pipeline_config(w, args, lanes, args.steps, args.warmup,
                                "mixed_pairs", parity=not args.no_parity, cpu=not args.no_cpu)
     log("headline: %.3f ms/step" % head["ms_per_step"])
@@ -572,9 +736,18 @@ def main():
                    "pairs": w.n_pairs, "seed": args.seed, "hull_vertices": "log-uniform 8..4096",
                    "lanes_per_pair": lanes or list(_native.DEFAULT_LANES)},
         "roofline": dict(head["roofline"], traffic=traffic_bytes(),
+                         traffic_source=("profiles/traffic_mixed_pairs.md (rocprofv3 FETCH_SIZE "
+                                         "and WRITE_SIZE passes of this workload, scaled by a "
+                                         "copy kernel of known size)"
+                                         if traffic_bytes() is not None else None),
                          kernel="gjk_epa pipeline step (counting sort + gjk_all_kernel + "
-                                "epa_kernel)", kernel_ms=head["event_ms_per_step"]),
+                                "epa_kernel)", kernel_ms=head["event_ms_per_step"],
+                         device_copy=copy_bw,
+                         frac_of_copy=round(head["roofline"]["achieved"] / copy_bw["GBps"], 5)),
     }
+    for k in ("gjk_iterations", "effective_sweep", "epa_status_counts", "epa_unconverged"):
+        if k in head:
+            result[k] = head[k]
     if "parity" in head:
         result["parity"] = head["parity"]
     if "cpu_baseline" in head:

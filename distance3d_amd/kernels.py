@@ -479,10 +479,11 @@ class GjkEpaResult:
 
 def gjk_epa_batch(scene, pair_a, pair_b, out=None, workspace=None, lanes_per_pair=None,
                   epa_max_iterations=64, epa_max_faces=128, epa_epsilon=1e-8, stream=None,
-                  **gjk_kw):
+                  iterations=None, **gjk_kw):
     """GJK distance for every pair and EPA for the intersecting ones, in one
     device pipeline (no host synchronisation).  ``epa_status`` is -1 where
-    GJK found no intersection."""
+    GJK found no intersection.  ``iterations`` (optional int32 device tensor
+    of n entries) receives each pair's GJK iteration count."""
     import ctypes
     import torch
     a = _dev_i32(pair_a, scene.device)
@@ -496,6 +497,6 @@ def gjk_epa_batch(scene, pair_a, pair_b, out=None, workspace=None, lanes_per_pai
     check(lib().d3d_gjk_epa_batch_ws(
         scene.c, ptr(a), ptr(b), n, ctypes.byref(prm), int(epa_max_iterations),
         int(epa_max_faces), float(epa_epsilon), ptr(out.dist), ptr(out.cp_a), ptr(out.cp_b),
-        ptr(out.status), None, ptr(out.epa_out), ptr(out.epa_status), ptr(workspace),
+        ptr(out.status), ptr(iterations), ptr(out.epa_out), ptr(out.epa_status), ptr(workspace),
         workspace.numel(), stream_ptr(stream)), "d3d_gjk_epa_batch_ws")
     return out

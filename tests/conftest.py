@@ -3,6 +3,16 @@ import sys
 
 import pytest
 
+# HIP places a process's streams round-robin on GPU_MAX_HW_QUEUES hardware
+# queues (4 by default); the serving tests keep four slot streams beside the
+# default stream, and slots sharing a queue run their batches one after the
+# other.  Raised (never lowered) before anything initialises HIP.
+try:
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
+except ValueError:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
