@@ -49,6 +49,7 @@ class d3d_scene(ctypes.Structure):
 
 ABI_VERSION = 3
 CLUSTER_SIZE = 16
+NOALLOC_MAX_PAIRS = 16384  # D3D_NOALLOC_MAX_PAIRS
 
 
 class d3d_gjk_params(ctypes.Structure):
@@ -80,6 +81,8 @@ _SIGNATURES = {
     "d3d_epa_batch": [_SCENE, _P, _P, _I64, _P, _P, _I32, _I32, _D, _P, _P, _P, _P, _P, _P],
     "d3d_gjk_epa_batch_ws": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params), _I32, _I32,
                              _D, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
+    "d3d_gjk_epa_batch_noalloc": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params), _I32,
+                                  _I32, _D, _P, _P, _P, _P, _P, _P, _P, _P],
     "d3d_instance_pairs": [_SCENE, _P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P],
     "d3d_bvh_build": [_P, _I64, _P, _I64, _P],
     "d3d_bvh_query": [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P],

@@ -33,5 +33,5 @@ extern "C" int d3d_init(int device) {
         d3d::set_error(std::string("libd3d_hip is built for gfx950, device is ") + prop.gcnArchName);
         return D3D_E_INVALID;
     }
-    return D3D_OK;
+    return d3d::noalloc_scratch(nullptr);
 }

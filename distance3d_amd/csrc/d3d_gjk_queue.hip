@@ -60,7 +60,7 @@ struct QueueWs {
     int32_t *hist, *key_off, *key_cur, *cls_begin, *cls_count, *cls_cursor, *order;
 };
 
-__host__ __device__ inline int64_t ws_header_ints() {
+__host__ __device__ constexpr int64_t ws_header_ints() {
     return ((3 * kNumKeys + 3 * kNumClasses) + 63) / 64 * 64;
 }
 
@@ -798,11 +798,17 @@ extern "C" int d3d_gjk_distance_batch_ws(const d3d_scene *scene, const int32_t *
 
 // Workspace of the fused pipeline: GJK queue | simplex (n*36 f64) |
 // n_points (n) | EPA work list (n) | count (u64)
-static inline int64_t a256(int64_t b) { return (b + 255) / 256 * 256; }
+static constexpr int64_t a256(int64_t b) { return (b + 255) / 256 * 256; }
+static constexpr int64_t gjk_ws_bytes(int64_t n) {
+    return (ws_header_ints() + n) * (int64_t)sizeof(int32_t);
+}
+static constexpr int64_t gjk_epa_ws_bytes(int64_t n) {
+    return a256(gjk_ws_bytes(n)) + a256(n * 36 * 8) + a256(n * 4) + a256(n * 4) + 256;
+}
 
 extern "C" int64_t d3d_gjk_epa_workspace_bytes(int64_t n) {
     if (n < 0) return -1;
-    return a256(d3d_gjk_workspace_bytes(n)) + a256(n * 36 * 8) + a256(n * 4) + a256(n * 4) + 256;
+    return gjk_epa_ws_bytes(n);
 }
 
 extern "C" int d3d_gjk_epa_batch_ws(const d3d_scene *scene, const int32_t *pair_a,
@@ -856,4 +862,65 @@ extern "C" int d3d_instance_pairs(const d3d_scene *scene, const int32_t *pair_a,
         D3D_CHECK_LAUNCH("iota_pairs_kernel");
     }
     return D3D_OK;
+}
+
+// ------------------------------------ fused pipeline without allocation
+
+// The fused pipeline's workspace for at most D3D_NOALLOC_MAX_PAIRS pairs, as
+// a device array of the library: each GPU holds its own copy from module load
+// until the process ends, and every call of d3d_gjk_epa_batch_noalloc on that
+// GPU works in it (the queue header and the EPA counters are cleared by fill
+// kernels at the start of each call; the other regions are written before
+// they are read).  In host code `scratch` names a host-side stand-in: kernels
+// get the device address that hipGetSymbolAddress returns for the current
+// device, kept per device in g_scratch.
+static constexpr int64_t kNoallocBytes = gjk_epa_ws_bytes(D3D_NOALLOC_MAX_PAIRS);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices];
+}  // namespace
+
+namespace d3d {
+
+// Looks up the device address of `scratch` on the current device once (the
+// device must be current); d3d_init calls it, and so does the first
+// d3d_gjk_epa_batch_noalloc call on a device d3d_init was not called for.
+int noalloc_scratch(void **out) {
+    int dev = 0;
+    D3D_CHECK_HIP(hipGetDevice(&dev));
+    D3D_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device index beyond the 64 the library tracks");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        D3D_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        D3D_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned NULL for the scratch array");
+        g_scratch[dev] = p;
+    }
+    if (out) *out = g_scratch[dev];
+    return D3D_OK;
+}
+
+}  // namespace d3d
+
+extern "C" int d3d_gjk_epa_batch_noalloc(const d3d_scene *scene, const int32_t *pair_a,
+                                         const int32_t *pair_b, int64_t n,
+                                         const d3d_gjk_params *params, int32_t epa_max_iterations,
+                                         int32_t epa_max_faces, double epa_epsilon, double *dist,
+                                         double *cp_a, double *cp_b, int32_t *status,
+                                         int32_t *iterations, double *epa_out,
+                                         int32_t *epa_status, void *stream) {
+    clear_error();
+    D3D_CHECK_ARG(n >= 0, "n must be >= 0");
+    D3D_CHECK_ARG(n <= D3D_NOALLOC_MAX_PAIRS,
+                  "n exceeds D3D_NOALLOC_MAX_PAIRS (16384 pairs per call of "
+                  "d3d_gjk_epa_batch_noalloc); split the batch or use d3d_gjk_epa_batch_ws");
+    if (n == 0) return D3D_OK;
+    void *ws = nullptr;
+    if (int rc = noalloc_scratch(&ws)) return rc;
+    return d3d_gjk_epa_batch_ws(scene, pair_a, pair_b, n, params, epa_max_iterations,
+                                epa_max_faces, epa_epsilon, dist, cp_a, cp_b, status, iterations,
+                                epa_out, epa_status, ws, kNoallocBytes, stream);
 }

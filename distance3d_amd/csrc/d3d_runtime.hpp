@@ -80,4 +80,8 @@ int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t
                unsigned long long *next, hipStream_t stream, double *faces = nullptr,
                int32_t *n_faces = nullptr);
 
+// Device address of d3d_gjk_epa_batch_noalloc's scratch array on the current
+// device (looked up once per device with hipGetSymbolAddress); out may be NULL.
+int noalloc_scratch(void **out);
+
 }  // namespace d3d

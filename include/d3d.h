@@ -259,6 +259,28 @@ int d3d_gjk_epa_batch_ws(const d3d_scene *scene, const int32_t *pair_a, const in
                          void *stream);
 
 /*
+ * d3d_gjk_epa_batch_ws with no workspace argument, allocating nothing: no
+ * hipMalloc, hipFree or allocating library call.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *   The workspace is an array inside the library of
+ *   d3d_gjk_epa_workspace_bytes(D3D_NOALLOC_MAX_PAIRS) bytes (about 5 MB),
+ *   one copy per GPU, held until the process ends.  n above
+ *   D3D_NOALLOC_MAX_PAIRS is rejected with D3D_E_INVALID.
+ *   Every call on one GPU works in the same array and it keeps the last
+ *   call's contents: calls that overlap in time on one GPU (other streams,
+ *   other host threads) race and must be serialised by the caller.
+ *   Outputs as d3d_gjk_epa_batch_ws (gjk/_gjk_jolt.py:138-221, epa.py:9-78).
+ */
+#define D3D_NOALLOC_MAX_PAIRS 16384
+int d3d_gjk_epa_batch_noalloc(const d3d_scene *scene, const int32_t *pair_a,
+                              const int32_t *pair_b, int64_t n, const d3d_gjk_params *params,
+                              int32_t epa_max_iterations, int32_t epa_max_faces,
+                              double epa_epsilon, double *dist, double *cp_a, double *cp_b,
+                              int32_t *status, int32_t *iterations, double *epa_out,
+                              int32_t *epa_status, void *stream);
+
+/*
  * Pose-updated pair instances (collider.update_pose per pair,
  * colliders.py:174-178, 276-279, 330-333, ...): writes a collider table of
  * 2n records -- record 2i = scene collider pair_a[i] with pose row

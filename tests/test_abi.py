@@ -81,3 +81,13 @@ def test_argument_errors_without_gpu():
     assert lib.d3d_epa_batch(*args, 64, 129, 1e-8, fake, fake, None, None, None, None) == invalid
     assert b"max_faces" in lib.d3d_last_error()
     assert lib.d3d_epa_batch(*args, 64, 3, 1e-8, fake, fake, None, None, None, None) == invalid
+    # allocation-free pipeline: sizes beyond its array are rejected, n == 0 is
+    # a no-op, both before any HIP call
+    big = _native.NOALLOC_MAX_PAIRS + 1
+    noalloc = [ctypes.byref(sc), fake, fake, big, None, 64, 128, 1e-8] + [fake] * 7 + [None]
+    assert lib.d3d_gjk_epa_batch_noalloc(*noalloc) == invalid
+    assert b"D3D_NOALLOC_MAX_PAIRS" in lib.d3d_last_error()
+    noalloc[3] = -1
+    assert lib.d3d_gjk_epa_batch_noalloc(*noalloc) == invalid
+    noalloc[3] = 0
+    assert lib.d3d_gjk_epa_batch_noalloc(*noalloc) == 0
