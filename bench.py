@@ -67,36 +67,53 @@ def parse():
     return p.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "traffic_mixed_pairs.json")
+KERNEL_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                              "gjk_epa_mixed_pairs_counters.json")
 
 
-def measured_traffic():
-    """HBM bytes per step of the headline workload as measured by the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in profiles/ (scaled by
-    the copy-kernel calibration stated there), or None."""
-    if not os.path.exists(TRAFFIC_FILE):
+def kernel_profile():
+    """Per-launch counters of the headline step's kernels from the rocprofv3
+    passes summarised in profiles/ (HBM bytes from FETCH_SIZE / WRITE_SIZE
+    scaled by a copy kernel of known size; FP64 VALU lane operations), or
+    None."""
+    if not os.path.exists(KERNEL_PROFILE):
         return None
-    with open(TRAFFIC_FILE) as f:
-        t = json.load(f)
-    return t
-
-
-def traffic_bytes():
-    t = measured_traffic()
-    return None if t is None else t["bytes_per_step"]
+    with open(KERNEL_PROFILE) as f:
+        return json.load(f)
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def timed_steps(step, steps, warmup):
-    """Runs warmup + timed steps; returns (wall ms/step, event ms per step)."""
+def set_stage_timing(enable):
+    from distance3d_amd import _native
+    _native.check(_native.lib().d3d_stage_timing(1 if enable else 0), "d3d_stage_timing")
+
+
+def stage_timing_read():
+    """Mean per-call milliseconds of the GJK pipeline's stages recorded since
+    stage timing was enabled (HIP events the library records on the launch
+    stream): counting sort, gjk_all_kernel, epa_kernel."""
+    import ctypes
+    from distance3d_amd import _native
+    ms = (ctypes.c_double * 3)()
+    calls = ctypes.c_int64(0)
+    _native.check(_native.lib().d3d_stage_timing_read(ms, ctypes.byref(calls)),
+                  "d3d_stage_timing_read")
+    k = max(calls.value, 1)
+    return {"sort_ms": ms[0] / k, "gjk_ms": ms[1] / k, "epa_ms": ms[2] / k, "calls": calls.value}
+
+
+def timed_steps(step, steps, warmup, stages=False):
+    """Runs warmup + timed steps; returns (wall ms/step, event ms per step)
+    and, with ``stages``, the library's per-stage times as well."""
     import torch
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
+    if stages:
+        set_stage_timing(True)
     stream = torch.cuda.current_stream()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(steps)]
@@ -108,7 +125,12 @@ def timed_steps(step, steps, warmup):
         ev[k][1].record(stream)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
-    return wall, np.array([a.elapsed_time(b) for a, b in ev])
+    ev_ms = np.array([a.elapsed_time(b) for a, b in ev])
+    if stages:
+        st = stage_timing_read()
+        set_stage_timing(False)
+        return wall, ev_ms, st
+    return wall, ev_ms
 
 
 def device_copy_bandwidth(mib=1024, reps=10):
@@ -227,7 +249,8 @@ def epa_parity(gpu_out, gpu_status, ref):
     }
 
 
-def run_pipeline_config(w, args, lanes, steps, warmup, label, parity=True, cpu=False):
+def run_pipeline_config(w, args, lanes, steps, warmup, label, parity=True, cpu=False,
+                        launch=None):
     """GJK+EPA pipeline over a workload: timing, roofline, parity, CPU."""
     import torch
     from distance3d_amd import kernels
@@ -239,14 +262,15 @@ def run_pipeline_config(w, args, lanes, steps, warmup, label, parity=True, cpu=F
     ws = torch.empty(kernels.gjk_epa_workspace_bytes(n), dtype=torch.uint8, device="cuda")
 
     def step():
-        kernels.gjk_epa_batch(dev, pa, pb, out=out, workspace=ws, lanes_per_pair=lanes)
+        kernels.gjk_epa_batch(dev, pa, pb, out=out, workspace=ws, lanes_per_pair=lanes,
+                              launch=launch)
 
-    wall, ev_ms = timed_steps(step, steps, warmup)
+    wall, ev_ms, stages = timed_steps(step, steps, warmup, stages=True)
     value = n / (wall / 1e3)
     # one more (untimed) pass for the per-pair GJK iteration counts
     iters = torch.empty(n, dtype=torch.int32, device="cuda")
     kernels.gjk_epa_batch(dev, pa, pb, out=out, workspace=ws, lanes_per_pair=lanes,
-                          iterations=iters)
+                          launch=launch, iterations=iters)
     iters = iters.cpu().numpy().astype(np.int64)
     dist = out.dist.cpu().numpy()
     hit = dist == 0.0
@@ -255,14 +279,26 @@ def run_pipeline_config(w, args, lanes, steps, warmup, label, parity=True, cpu=F
     hull_bytes = w.hull_bytes_per_pair()
     bytes_step = float(np.sum(hull_bytes) + n * (128 + 8 + 60) + 80 * hit.sum())
     achieved = bytes_step / (ev_ms.mean() / 1e3) / 1e9
+    # the GJK kernel alone: hull bytes, records, pair ids and GJK outputs
+    gjk_bytes = float(np.sum(hull_bytes) + n * (128 + 8 + 60))
+    gjk_ms = stages["gjk_ms"]
+    gjk_achieved = gjk_bytes / (gjk_ms / 1e3) / 1e9
     # every GJK iteration evaluates both supports: a full sweep of both hulls
     # would read 12 (n_A + n_B) bytes
     sweep_bytes = float(np.sum(iters * hull_bytes))
     res = {"workload": label, "pairs": n, "value": round(value, 1), "unit": "pair-queries/s",
            "ms_per_step": round(wall, 4), "event_ms_per_step": round(float(ev_ms.mean()), 4),
-           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                        "bytes_per_launch": bytes_step},
+           "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+           "roofline": {"bound": "hbm", "kernel": "gjk_all_kernel",
+                        "achieved": round(gjk_achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gjk_achieved / HBM_PEAK_GBS, 5),
+                        "bytes_per_launch": gjk_bytes, "kernel_ms": round(gjk_ms, 4),
+                        "timing": "HIP events recorded by the library on the launch stream "
+                                  "around gjk_all_kernel, averaged over the timed steps",
+                        "step": {"achieved": round(achieved, 2),
+                                 "frac": round(achieved / HBM_PEAK_GBS, 5),
+                                 "bytes_per_step": bytes_step,
+                                 "ms": round(float(ev_ms.mean()), 4)}},
            "gjk_iterations": {"mean": round(float(iters.mean()), 3), "max": int(iters.max()),
                               "total": int(iters.sum())},
            "effective_sweep": {"bytes": sweep_bytes,
@@ -637,7 +673,7 @@ def run_contact(args, steps=10, warmup=2):
     return res
 
 
-def class_breakdown(w, lanes, reps=5):
+def class_breakdown(w, lanes, reps=5, launch=None):
     """Each work class of the workload (pairs sorted by their total hull
     vertex count, 8 classes) as a GJK batch of its own: event time, pairs,
     GJK iterations, and the algorithmic bytes 12 (n_A + n_B) + 196 per pair
@@ -664,7 +700,8 @@ def class_breakdown(w, lanes, reps=5):
         ws = torch.empty(kernels.gjk_workspace_bytes(n), dtype=torch.uint8, device="cuda")
 
         def step():
-            kernels.gjk_distance_batch(dev, pa, pb, lanes_per_pair=lanes, out=out, workspace=ws)
+            kernels.gjk_distance_batch(dev, pa, pb, lanes_per_pair=lanes, out=out, workspace=ws,
+                                       launch=launch)
 
         log("class %d: %d pairs" % (c, n))
         wall, ev = timed_steps(step, reps, 1)
@@ -696,6 +733,7 @@ def main():
 
     from distance3d_amd import autotune
     lanes = [int(x) for x in args.lanes.split(",")] if args.lanes else autotune.tuned_lanes()
+    launch = autotune.tuned_launch()
     if args.copy_calibration_mib > 0:
         src = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -711,12 +749,12 @@ def main():
     log("workload generated in %.1fs: %d pairs, %d hull vertices"
         % (time.time() - t0, w.n_pairs, w.scene.n_vertices))
     if args.classes:
-        print(json.dumps({"class_breakdown": class_breakdown(w, lanes),
+        print(json.dumps({"class_breakdown": class_breakdown(w, lanes, launch=launch),
                           "workload": "mixed_pairs seed %d, %d pairs" % (args.seed, w.n_pairs)}),
               flush=True)
         return
-    head = run_pipeline_config(w, args, lanes, args.steps, args.warmup,
-                               "mixed_pairs", parity=not args.no_parity, cpu=not args.no_cpu)
+    head = run_pipeline_config(w, args, lanes, args.steps, args.warmup, "mixed_pairs",
+                               parity=not args.no_parity, cpu=not args.no_cpu, launch=launch)
     log("headline: %.3f ms/step" % head["ms_per_step"])
     result = {
         "metric": METRIC,
@@ -734,17 +772,32 @@ def main():
         "config": {"workload": "mixed_pairs: 1,048,576 mixed-primitive pairs, GJK + EPA on "
                                "intersecting pairs",
                    "pairs": w.n_pairs, "seed": args.seed, "hull_vertices": "log-uniform 8..4096",
-                   "lanes_per_pair": lanes or list(_native.DEFAULT_LANES)},
-        "roofline": dict(head["roofline"], traffic=traffic_bytes(),
-                         traffic_source=("profiles/traffic_mixed_pairs.md (rocprofv3 FETCH_SIZE "
-                                         "and WRITE_SIZE passes of this workload, scaled by a "
-                                         "copy kernel of known size)"
-                                         if traffic_bytes() is not None else None),
-                         kernel="gjk_epa pipeline step (counting sort + gjk_all_kernel + "
-                                "epa_kernel)", kernel_ms=head["event_ms_per_step"],
-                         device_copy=copy_bw,
+                   "lanes_per_pair": lanes or list(_native.DEFAULT_LANES),
+                   "launch": launch or dict(autotune.LAUNCH_DEFAULT)},
+        "roofline": dict(head["roofline"], device_copy=copy_bw,
                          frac_of_copy=round(head["roofline"]["achieved"] / copy_bw["GBps"], 5)),
     }
+    prof = kernel_profile()
+    stages = head["stages_ms"]
+    rl = result["roofline"]
+    rl["traffic"] = None
+    if prof is not None:
+        g = prof["kernels"]["gjk_all_kernel"]
+        rl["traffic"] = g["fetch_bytes"] + g["write_bytes"]
+        rl["traffic_source"] = ("profiles/gjk_epa_mixed_pairs_counters.md: rocprofv3 FETCH_SIZE and "
+                                "WRITE_SIZE passes of this workload, gjk_all_kernel per launch, "
+                                "scaled by a copy kernel of known size")
+        rl["traffic_over_algorithmic"] = round(rl["traffic"] / rl["bytes_per_launch"], 3)
+        fp = {}
+        for name, ms in (("gjk_all_kernel", stages["gjk_ms"]), ("epa_kernel", stages["epa_ms"])):
+            ops = prof["kernels"][name]["fp64_lane_ops"]
+            tf = ops / (ms / 1e3) / 1e12
+            fp[name] = {"lane_ops_per_launch": ops, "achieved": round(tf, 3), "peak": 78.6,
+                        "unit": "TFLOP/s", "frac": round(tf / 78.6, 5)}
+        result["fp64_valu"] = dict(fp, note=(
+            "FP64 VALU operations per launch from rocprofv3 (SQ_INSTS_VALU_ADD/MUL/TRANS_F64 + "
+            "2 x FMA_F64, x 64 lanes: issued lanes, an upper bound on useful FLOPs) over the "
+            "kernel time measured in this run"))
     for k in ("gjk_iterations", "effective_sweep", "epa_status_counts", "epa_unconverged"):
         if k in head:
             result[k] = head[k]
@@ -768,14 +821,14 @@ def main():
         if "hull" in only:
             cfgs["hull_pairs"] = run_pipeline_config(workloads.hull_pairs(), args, lanes, 20, 3,
                                                      "hull_pairs (4096 x 32-vertex hulls)",
-                                                     parity=not args.no_parity)
+                                                     parity=not args.no_parity, launch=launch)
         if "epa" in only:
             t0 = time.time()
             we = workloads.epa_pairs()
             log("epa workload in %.1fs" % (time.time() - t0))
             cfgs["epa_pairs"] = run_pipeline_config(we, args, lanes, 5, 1,
                                                     "epa_pairs (262,144 overlapping hull pairs)",
-                                                    parity=not args.no_parity)
+                                                    parity=not args.no_parity, launch=launch)
         if "broad" in only:
             cfgs["broad_phase"] = run_broad(args)
         if "serving" in only:

@@ -47,7 +47,7 @@ class d3d_scene(ctypes.Structure):
     ]
 
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 CLUSTER_SIZE = 16
 NOALLOC_MAX_PAIRS = 16384  # D3D_NOALLOC_MAX_PAIRS
 
@@ -60,6 +60,9 @@ class d3d_gjk_params(ctypes.Structure):
         ("max_iterations", ctypes.c_int32),
         ("hull_serial_max", ctypes.c_int32),
         ("lanes_per_pair", ctypes.c_int32 * 8),
+        ("blocks_per_cu", ctypes.c_int32),
+        ("claim_factor", ctypes.c_int32),
+        ("epa_blocks_per_cu", ctypes.c_int32),
     ]
 
 
@@ -83,6 +86,8 @@ _SIGNATURES = {
                              _D, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P],
     "d3d_gjk_epa_batch_noalloc": [_SCENE, _P, _P, _I64, ctypes.POINTER(d3d_gjk_params), _I32,
                                   _I32, _D, _P, _P, _P, _P, _P, _P, _P, _P],
+    "d3d_stage_timing": [_I32],
+    "d3d_stage_timing_read": [_P, _P],
     "d3d_instance_pairs": [_SCENE, _P, _P, _P, _P, _P, _P, _I64, _I64, _P, _P, _P, _P, _P],
     "d3d_bvh_build": [_P, _I64, _P, _I64, _P],
     "d3d_bvh_query": [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P],
@@ -179,7 +184,10 @@ DEFAULT_LANES = (1, 1, 1, 1, 2, 4, 8, 16)
 
 
 def gjk_params(tolerance=1e-10, max_distance_squared=100000.0, sanity_check=1e-8,
-               max_iterations=128, hull_serial_max=16, lanes_per_pair=None):
+               max_iterations=128, hull_serial_max=16, lanes_per_pair=None, launch=None):
+    """d3d_gjk_params.  ``launch``: optional dict of the launch shape
+    (``blocks_per_cu``, ``claim_factor``, ``epa_blocks_per_cu``; 0 or absent =
+    the library default)."""
     p = d3d_gjk_params()
     p.tolerance = float(tolerance)
     p.max_distance_squared = float(max_distance_squared)
@@ -191,6 +199,13 @@ def gjk_params(tolerance=1e-10, max_distance_squared=100000.0, sanity_check=1e-8
         raise ValueError("lanes_per_pair needs 8 entries (one per work class)")
     for c, g in enumerate(lanes):
         p.lanes_per_pair[c] = int(g)
+    launch = launch or {}
+    unknown = set(launch) - {"blocks_per_cu", "claim_factor", "epa_blocks_per_cu"}
+    if unknown:
+        raise ValueError("unknown launch keys: %s" % sorted(unknown))
+    p.blocks_per_cu = int(launch.get("blocks_per_cu", 0))
+    p.claim_factor = int(launch.get("claim_factor", 0))
+    p.epa_blocks_per_cu = int(launch.get("epa_blocks_per_cu", 0))
     return p
 
 

@@ -1,18 +1,27 @@
-"""Launch tuning for the batched GJK: lanes per pair for each work class.
+"""Launch tuning for the batched GJK: lanes per pair for each work class and
+the launch shape (the tile of pairs a wavefront claims, blocks per compute
+unit).
 
 The GJK pipeline sorts pairs into 8 classes by their total hull vertex count
-(0, <=32, <=128, <=512, <=1024, <=2048, <=4096, >4096).  Two tables are kept in
+(0, <=32, <=128, <=512, <=1024, <=2048, <=4096, >4096).  The tables kept in
 ``tuning.json`` beside this file:
 
 * ``lanes_per_pair`` (throughput, :func:`tune`): for each class, the GJK
   kernels are timed on that class's own pairs (taken from a large sample
   workload) with every candidate lane count and the fastest is kept;
+* ``launch`` (throughput, :func:`tune_launch`): coordinate descent over the
+  launch shape -- ``blocks_per_cu`` persistent GJK blocks per compute unit,
+  ``claim_factor`` (a wavefront claims claim_factor * 64 / G pairs of a class
+  at a time: the tile of pairs it works through), ``epa_blocks_per_cu``
+  persistent EPA wavefronts per compute unit -- minimising the whole GJK + EPA
+  step;
 * ``serving_lanes_per_pair`` (latency, :func:`tune_latency`): coordinate
   descent over the classes, minimising the median time of one GJK + EPA
   batch of a single large request on an otherwise idle GPU -- more lanes per
   hull pair shorten each support sweep, which is what bounds a small batch.
 
-:func:`tuned_lanes` returns either table.
+:func:`tuned_lanes` and :func:`tuned_launch` return the tables.  The selection
+(:func:`pick`, :func:`descend`) is plain host logic over measured times.
 
 Usage: ``python -m distance3d_amd.autotune [--pairs N] [--serving-requests M]``
 on the GPU.
@@ -32,6 +41,43 @@ CANDIDATES = {0: (1, 2, 4), 1: (1, 2, 4, 8), 2: (2, 4, 8, 16), 3: (4, 8, 16, 32)
               4: (8, 16, 32, 64), 5: (8, 16, 32, 64), 6: (16, 32, 64), 7: (16, 32, 64)}
 LATENCY_CANDIDATES = {0: (1, 2, 4), 1: (4, 8, 16, 32), 2: (8, 16, 32, 64), 3: (16, 32, 64),
                       4: (16, 32, 64), 5: (32, 64), 6: (64,), 7: (64,)}
+
+
+LAUNCH_CANDIDATES = {"blocks_per_cu": (2, 3), "claim_factor": (1, 2, 4),
+                     "epa_blocks_per_cu": (4, 8, 12)}
+LAUNCH_DEFAULT = {"blocks_per_cu": 3, "claim_factor": 2, "epa_blocks_per_cu": 8}
+
+
+def pick(times):
+    """The candidate with the smallest time; on equal times the one listed
+    first (``times`` is an ordered mapping candidate -> time)."""
+    best = None
+    for k, t in times.items():
+        if best is None or t < times[best]:
+            best = k
+    return best
+
+
+def descend(measure, start, candidates, sweeps=1, log=None):
+    """Coordinate descent: for each key of ``candidates`` in order, every
+    candidate value is measured with the other keys held at their current
+    values and the fastest is kept; ``sweeps`` passes over the keys.
+    ``measure(config) -> time``.  Returns (config, report)."""
+    cfg = dict(start)
+    report = {"start": {"config": dict(cfg), "time": measure(dict(cfg))}}
+    for sw in range(sweeps):
+        for key, values in candidates.items():
+            times = {}
+            for v in values:
+                trial = dict(cfg)
+                trial[key] = v
+                times[v] = measure(trial)
+            cfg[key] = pick(times)
+            report["sweep%d_%s" % (sw, key)] = {str(v): t for v, t in times.items()}
+            if log:
+                log("%s: %s -> %s" % (key, times, cfg[key]))
+    report["final"] = {"config": dict(cfg), "time": measure(dict(cfg))}
+    return cfg, report
 
 
 def work_class(nverts):
@@ -59,6 +105,16 @@ def tuned_lanes(path=TUNING_PATH, kind="throughput"):
         t = json.load(f)
     lanes = t.get("lanes_per_pair" if kind == "throughput" else "serving_lanes_per_pair")
     return None if lanes is None else [int(x) for x in lanes]
+
+
+def tuned_launch(path=TUNING_PATH, kind="throughput"):
+    """The tuned launch shape (dict for ``launch=``), or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    launch = t.get("launch" if kind == "throughput" else "serving_launch")
+    return None if launch is None else {k: int(v) for k, v in launch.items()}
 
 
 def _time(dev, pa, pb, lanes, reps):
@@ -98,7 +154,7 @@ def tune(workload, reps=3, log=print):
             trial = list(lanes)
             trial[c] = g
             times[g] = _time(dev, pa, pb, trial, reps)
-        best = min(times, key=times.get)
+        best = pick(times)
         lanes[c] = best
         report[c] = {"pairs": int(len(idx)), "ms": {str(k): round(v * 1e3, 4) for k, v in
                                                     times.items()}, "best": best}
@@ -106,26 +162,52 @@ def tune(workload, reps=3, log=print):
     return lanes, report
 
 
-def combined_step_ms(workload, lanes, reps=5):
-    """Best time of the whole GJK + EPA step over every pair of the workload
-    (all classes in one launch, as bench.py runs it)."""
-    import torch
-    from . import kernels
-    dev = workload.scene.to_device()
-    pa = torch.from_numpy(workload.pair_a).cuda()
-    pb = torch.from_numpy(workload.pair_b).cuda()
-    n = pa.numel()
-    out = kernels.GjkEpaResult.empty(n, "cuda")
-    ws = torch.empty(kernels.gjk_epa_workspace_bytes(n), dtype=torch.uint8, device="cuda")
-    kernels.gjk_epa_batch(dev, pa, pb, out=out, workspace=ws, lanes_per_pair=lanes)
-    torch.cuda.synchronize()
-    best = float("inf")
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        kernels.gjk_epa_batch(dev, pa, pb, out=out, workspace=ws, lanes_per_pair=lanes)
+class StepTimer:
+    """Times the whole GJK + EPA step over every pair of a workload (all
+    classes in one launch, as bench.py runs it); the device scene and
+    buffers are built once."""
+
+    def __init__(self, workload):
+        import torch
+        from . import kernels
+        self.kernels = kernels
+        self.dev = workload.scene.to_device()
+        self.pa = torch.from_numpy(workload.pair_a).cuda()
+        self.pb = torch.from_numpy(workload.pair_b).cuda()
+        n = self.pa.numel()
+        self.out = kernels.GjkEpaResult.empty(n, "cuda")
+        self.ws = torch.empty(kernels.gjk_epa_workspace_bytes(n), dtype=torch.uint8,
+                              device="cuda")
+
+    def __call__(self, lanes, launch=None, reps=5):
+        """Best of ``reps`` step times in ms."""
+        import torch
+        run = lambda: self.kernels.gjk_epa_batch(self.dev, self.pa, self.pb, out=self.out,
+                                                 workspace=self.ws, lanes_per_pair=lanes,
+                                                 launch=launch)
+        run()
         torch.cuda.synchronize()
-        best = min(best, time.perf_counter() - t0)
-    return 1e3 * best
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return 1e3 * best
+
+
+def combined_step_ms(workload, lanes, reps=5, launch=None):
+    """Best time of the whole GJK + EPA step over every pair of the workload."""
+    return StepTimer(workload)(lanes, launch, reps)
+
+
+def tune_launch(workload, lanes, sweeps=1, log=print, timer=None):
+    """Launch shape minimising the whole step (coordinate descent over
+    LAUNCH_CANDIDATES from LAUNCH_DEFAULT)."""
+    timer = timer or StepTimer(workload)
+    measure = lambda cfg: round(timer(lanes, cfg), 4)
+    return descend(measure, LAUNCH_DEFAULT, LAUNCH_CANDIDATES, sweeps=sweeps,
+                   log=lambda m: log("launch " + m))
 
 
 def tune_latency(scene, poses, requests, start=None, sweeps=2, reps=3, log=print):
@@ -168,7 +250,7 @@ def tune_latency(scene, poses, requests, start=None, sweeps=2, reps=3, log=print
                 trial = list(lanes)
                 trial[c] = g
                 times[g] = measure(trial)
-            lanes[c] = min(times, key=times.get)
+            lanes[c] = pick(times)
             report["sweep%d_class%d" % (sweep, c)] = {str(k): round(v * 1e3, 4) for k, v in
                                                       times.items()}
             log("latency sweep %d class %d: %s -> %d lanes" % (
@@ -190,24 +272,28 @@ def main():
     # classes overlap in the combined launch, so the per-class optimum is
     # checked against the installed table on the whole step; the faster stays
     installed = tuned_lanes()
+    timer = StepTimer(w)
     if installed is not None and installed != lanes:
-        t_new, t_old = combined_step_ms(w, lanes), combined_step_ms(w, installed)
+        t_new, t_old = timer(lanes), timer(installed)
         report["combined_ms"] = {"tuned": [lanes, round(t_new, 4)],
                                  "installed": [installed, round(t_old, 4)]}
         print("combined step: tuned %s %.3f ms, installed %s %.3f ms" % (lanes, t_new, installed, t_old))
         if t_old < t_new:
             lanes = installed
+    launch, lreport = tune_launch(w, lanes, timer=timer)
     scene, poses, reqs = workloads.serving_workload(n_requests=20000)
     big = [r for r in reqs if r.n_pairs >= 2048][:args.serving_requests]
     slanes, sreport = tune_latency(scene, poses, big, start=lanes)
     with open(args.out, "w") as f:
         json.dump({"lanes_per_pair": lanes, "workload": "mixed_pairs seed 1, %d pairs" % args.pairs,
                    "classes": report,
+                   "launch": launch, "launch_tuning": lreport,
                    "serving_lanes_per_pair": slanes,
                    "serving_workload": "%d requests of >= 2048 pairs from the serving stream "
                                        "(seed 4), one batch each on an idle GPU" % len(big),
                    "serving": sreport}, f, indent=1)
-    print(json.dumps({"lanes_per_pair": lanes, "serving_lanes_per_pair": slanes}))
+    print(json.dumps({"lanes_per_pair": lanes, "launch": launch,
+                      "serving_lanes_per_pair": slanes}))
 
 
 if __name__ == "__main__":

@@ -561,6 +561,54 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
     return hull_rescan<G>(s, H, l, sub, __double2float_rd((double)m1 - 2.5 * E));
 }
 
+// Argmax over a hull whose FP32 vertices sit in LDS in their ORIGINAL order
+// (cx/cy/cz[off + j] = vertex j; the hull's vertices are exact in FP32), by
+// the G lanes of a group: the sweep and certificate of hull_argmax, and, when
+// the certificate fails, the FP64 rescan of the candidates from the same LDS
+// copy with ties to the smallest (original) index.  Returns the original
+// index of vertices[argmax(vertices . l)] (colliders.py:131-132, first index
+// on ties), the same vertex hull_argmax finds in the pools.
+template <int G>
+__device__ __forceinline__ int lds_orig_argmax(const float *cx, const float *cy, const float *cz,
+                                               int off, int nv, double Lmax, v3 l, int sub) {
+    const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
+    const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
+    const double E = 8.0 * 5.9604644775390625e-08 * Lmax * linf;
+    float m1 = -INFINITY, m2 = -INFINITY;
+    int i1 = 0x7fffffff;
+    for (int j = sub; j < nv; j += G)
+        top2(__builtin_fmaf(cz[off + j], dz, __builtin_fmaf(cy[off + j], dy, cx[off + j] * dx)), j, m1,
+             m2, i1);
+    group_top2<G>(m1, m2, i1);
+    if ((double)m1 - (double)m2 > 2.0 * E) return i1;
+    float thr = __double2float_rd((double)m1 - 2.5 * E);
+    if (thr != thr) thr = -INFINITY;
+    ArgmaxF64 A;
+    A.init();
+    for (int j = sub; j < nv; j += G) {
+        const float x = cx[off + j], y = cy[off + j], z = cz[off + j];
+        if (!(__builtin_fmaf(z, dz, __builtin_fmaf(y, dy, x * dx)) < thr))
+            A.take((double)x * l.x + (double)y * l.y + (double)z * l.z, j, j);
+    }
+    return A.group_result<G>();
+}
+
+// Values of x in the lower and in the upper H-lane half of the lane's aligned
+// 2H-lane block, in every lane of the block (x uniform within each half).
+template <int H>
+__device__ __forceinline__ void halves(double x, double &lo, double &hi) {
+    double a, b;
+    xpair<H>(x, a, b);
+    if constexpr (H <= 8) {
+        const bool lower = (lane_id() & H) == 0;
+        lo = lower ? a : b;
+        hi = lower ? b : a;
+    } else {
+        lo = a;
+        hi = b;
+    }
+}
+
 // Closed-form supports, FP64, in the same operation order as the reference.
 __device__ __forceinline__ v3 support_analytic(const Shape &h, v3 d) {
     switch (h.type) {

@@ -69,41 +69,6 @@ __device__ __forceinline__ Shape lds_shape(const EpaLds &L, int k) {
     return h;
 }
 
-// Argmax over a hull whose FP32 vertices are cached in LDS at `off` (pool
-// order), by the G lanes of a group (G = 64: the wavefront; G = 32: one half,
-// the other half sweeping another hull at the same time; `sub` = lane index
-// in the group): the FP32 sweep and certificate of hull_argmax, with its FP64
-// rescan from the pools when the certificate fails.  All 64 lanes call it.
-template <int G>
-__device__ __forceinline__ int lds_hull_argmax(const d3d_scene &s, const EpaLds &L, int off,
-                                               const HullRef &H, v3 l, int sub) {
-    const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
-    const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
-    const double E = 8.0 * 5.9604644775390625e-08 * H.L * linf;
-    float m1 = -INFINITY, m2 = -INFINITY;
-    int i1 = 0x7fffffff;
-    for (int j = sub; j < H.nv; j += G)
-        top2(__builtin_fmaf(L.cz[off + j], dz, __builtin_fmaf(L.cy[off + j], dy, L.cx[off + j] * dx)), j,
-             m1, m2, i1);
-    group_top2<G>(m1, m2, i1);
-    const bool certified = (double)m1 - (double)m2 > 2.0 * E;
-    if (__ballot(!certified) == 0ull) return i1;
-    // the FP64 rescan of hull_rescan, with the candidates found in the cache
-    // (groups whose certificate held take no candidate and keep i1)
-    float thr = __double2float_rd((double)m1 - 2.5 * E);
-    if (thr != thr) thr = -INFINITY;
-    ArgmaxF64 A;
-    A.init();
-    if (!certified)
-        for (int j = sub; j < H.nv; j += G) {
-            const float sc =
-                __builtin_fmaf(L.cz[off + j], dz, __builtin_fmaf(L.cy[off + j], dy, L.cx[off + j] * dx));
-            if (!(sc < thr)) rescan_take(s, H, l, j, A);
-        }
-    const int r = A.group_result<G>();
-    return certified ? i1 : r;
-}
-
 __device__ __forceinline__ v3 readlane_v3(v3 v, int lane) {
     return mk(readlane_f64(v.x, lane), readlane_f64(v.y, lane), readlane_f64(v.z, lane));
 }
@@ -123,9 +88,8 @@ __device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, cons
         const v3 dir = vsel(half == 0, d, -d);
         const v3 l = to_local(h, dir);
         const int off = L.coff[half];
-        const int idx = lds_hull_argmax<32>(s, L, off, hull_ref(h), l, lane_id() & 31);
-        const v3 v = h.voff64 < 0 ? mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx])
-                                  : hull_vertex(s, h.voff, h.voff64, idx);
+        const int idx = lds_orig_argmax<32>(L.cx, L.cy, L.cz, off, h.nv, h.p[0], l, lane_id() & 31);
+        const v3 v = mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx]);
         const v3 out = apply_margin(h, to_world(h, v), dir);
         return SupportPair{readlane_v3(out, 0), readlane_v3(out, 32)};
     }
@@ -140,9 +104,8 @@ __device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, cons
             const int off = L.coff[k];
             v3 v;
             if (off >= 0) {
-                const int idx = lds_hull_argmax<kWave>(s, L, off, H, l, lane_id());
-                v = h.voff64 < 0 ? mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx])
-                                 : hull_vertex(s, h.voff, h.voff64, idx);
+                const int idx = lds_orig_argmax<kWave>(L.cx, L.cy, L.cz, off, h.nv, h.p[0], l, lane_id());
+                v = mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx]);
             } else {
                 v = hull_vertex(s, h.voff, h.voff64, hull_argmax<kWave>(s, H, l, lane_id()));
             }
@@ -393,21 +356,30 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             if (lane < 32) L.recf[lane >> 4][lane & 15] = s.shape_f64[16 * (lane < 16 ? ra : rb) + (lane & 15)];
             else if (lane < 40) L.reci[(lane - 32) >> 2][lane & 3] = s.shape_i32[4 * (lane < 36 ? ra : rb) + (lane & 3)];
         }
-        // ---- hull vertices into the LDS cache when both fit
+        // ---- hull vertices into the LDS cache when both fit, in their
+        // original order (so sweeps and their FP64 rescans read LDS only and
+        // ties go to the smallest index as in the pools); hulls whose vertices
+        // are not exact in FP32 are swept from the pools
         {
-            const int na = SA.type == D3D_HULL ? SA.nv : 0;
-            const int nb = SB.type == D3D_HULL ? SB.nv : 0;
+            const int na = SA.type == D3D_HULL && SA.voff64 < 0 ? SA.nv : 0;
+            const int nb = SB.type == D3D_HULL && SB.voff64 < 0 ? SB.nv : 0;
             const bool fit = na + nb <= kEpaCache;
             if (fit) {
                 for (int j = lane; j < na; j += kEpaBlock) {
-                    L.cx[j] = s.vx[SA.voff + j];
-                    L.cy[j] = s.vy[SA.voff + j];
-                    L.cz[j] = s.vz[SA.voff + j];
+                    const int o = s.vorig ? s.vorig[SA.voff + j] : j;
+                    if ((unsigned)o < (unsigned)na) {
+                        L.cx[o] = s.vx[SA.voff + j];
+                        L.cy[o] = s.vy[SA.voff + j];
+                        L.cz[o] = s.vz[SA.voff + j];
+                    }
                 }
                 for (int j = lane; j < nb; j += kEpaBlock) {
-                    L.cx[na + j] = s.vx[SB.voff + j];
-                    L.cy[na + j] = s.vy[SB.voff + j];
-                    L.cz[na + j] = s.vz[SB.voff + j];
+                    const int o = s.vorig ? s.vorig[SB.voff + j] : j;
+                    if ((unsigned)o < (unsigned)nb) {
+                        L.cx[na + o] = s.vx[SB.voff + j];
+                        L.cy[na + o] = s.vy[SB.voff + j];
+                        L.cz[na + o] = s.vz[SB.voff + j];
+                    }
                 }
             }
             if (lane == 0) {
@@ -718,14 +690,15 @@ int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t
                const int64_t *n_dev, const int32_t *work, const int32_t *work_off,
                const double *simplex, const int32_t *npts, int max_iter, int max_faces,
                double eps, double *out, int32_t *status, int32_t *iters,
-               unsigned long long *next, hipStream_t stream, double *faces, int32_t *n_faces) {
+               unsigned long long *next, hipStream_t stream, double *faces, int32_t *n_faces,
+               int blocks_per_cu) {
     if (n_max <= 0) return D3D_OK;
     int64_t grid = n_max;
     if (n_dev) {
         int cus = 256, dev = 0;
         if (hipGetDevice(&dev) == hipSuccess)
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        const int64_t cap = (int64_t)cus * 8;
+        const int64_t cap = (int64_t)cus * blocks_per_cu;
         if (grid > cap) grid = cap;
     } else if (grid > 65536) {
         grid = 65536;

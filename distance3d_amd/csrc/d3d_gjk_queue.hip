@@ -15,9 +15,11 @@
 //    pair terminates, so lanes do not idle on the slowest pair of a wave.
 // Counts and offsets stay on the device: no host synchronisation inside a
 // batch.
+#include <array>
 #include <map>
 #include <mutex>
 #include <utility>
+#include <vector>
 
 #include "d3d_device.hpp"
 #include "d3d_runtime.hpp"
@@ -409,11 +411,101 @@ __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 }
 
+// ---------------------------------------------- LDS-resident hull vertices
+//
+// A wavefront's 12 KB LDS region holds the P/Q rows of its 64/G pair slots
+// (24 doubles each, interleaved) and, for hull classes with G >= 4, one vertex
+// cache per slot in the rest: the FP32 vertices of the pair's hulls in their
+// original order (A's at 0, B's after them; x, y and z arrays of
+// cache_verts<G>() entries), loaded once when the group takes the pair.  Every
+// support sweep of the pair then reads LDS instead of the pools.  Slot
+// strides are congruent to G modulo 64 words, so the 64/G groups sweeping at
+// the same time hit distinct banks.  Pairs whose hulls do not fit, or whose
+// vertices are not exact in FP32, sweep the pools as before.
+template <int G>
+__host__ __device__ constexpr int cache_stride_words() {
+    // (12288 bytes - P/Q rows) / slots, rounded down to G modulo 64 words
+    constexpr int room = (3072 - 48 * (kWave / G)) / (kWave / G);
+    return G >= kWave ? room : ((room - G) / 64) * 64 + G;
+}
+template <int G>
+__host__ __device__ constexpr int cache_verts() {
+    return cache_stride_words<G>() / 3;
+}
+static_assert(cache_verts<4>() >= 32, "class 1 pairs fit a 4-lane slot");
+static_assert(cache_verts<64>() >= 1000, "one slot per wavefront holds 1000 vertices");
+
+// Copies the FP32 vertices of hull record h (validated) into the cache at
+// `at`, in original order (vorig), by the G lanes of a group.
+template <int G>
+__device__ __forceinline__ void cache_fill(const d3d_scene &s, const Shape &h, float *cache,
+                                           int at, int sub) {
+    constexpr int C = cache_verts<G>();
+    const int64_t base = h.voff;
+#pragma unroll 2
+    for (int j = sub; j < h.nv; j += G) {
+        const int o = s.vorig ? s.vorig[base + j] : j;
+        const float x = s.vx[base + j], y = s.vy[base + j], z = s.vz[base + j];
+        // an original index outside the hull (a malformed table) writes nothing
+        if ((unsigned)o < (unsigned)h.nv) {
+            cache[at + o] = x;
+            cache[C + at + o] = y;
+            cache[2 * C + at + o] = z;
+        }
+    }
+}
+
+// Supports of records ia along d and ib along -d (see support_pair) for a
+// pair whose hull vertices are in the group's cache (A's first, nA of them).
+// Two hulls: the lower half of the group sweeps A while the upper half sweeps
+// B; one hull: the whole group sweeps it.  Same results as support_pair.
+template <int G>
+__device__ __forceinline__ SupportPair support_pair_cached(const d3d_scene &s, int ia, int ib, v3 d,
+                                                           int sub, const float *cache, int nA,
+                                                           bool two) {
+    constexpr int C = cache_verts<G>();
+    const float *cx = cache, *cy = cache + C, *cz = cache + 2 * C;
+    if (two) {
+        constexpr int H = G / 2;
+        const bool upper = sub >= H;
+        const Shape h = load_shape(s, opaque(upper ? ib : ia));
+        const v3 dir = vsel(upper, -d, d);
+        const int off = upper ? nA : 0;
+        const int j = lds_orig_argmax<H>(cx, cy, cz, off, h.nv, h.p[0], to_local(h, dir), sub & (H - 1));
+        const v3 v = mk((double)cx[off + j], (double)cy[off + j], (double)cz[off + j]);
+        const v3 out = apply_margin(h, to_world(h, v), dir);
+        SupportPair r;
+        halves<H>(out.x, r.a.x, r.b.x);
+        halves<H>(out.y, r.a.y, r.b.y);
+        halves<H>(out.z, r.a.z, r.b.z);
+        return r;
+    }
+    v3 a = mk(0.0, 0.0, 0.0), b = a;
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const Shape h = load_shape(s, opaque(k == 0 ? ia : ib));
+        const v3 dir = vsel(k == 0, d, -d);
+        v3 out;
+        if (h.type == D3D_HULL) {
+            const int j = lds_orig_argmax<G>(cx, cy, cz, 0, h.nv, h.p[0], to_local(h, dir), sub);
+            out = to_world(h, mk((double)cx[j], (double)cy[j], (double)cz[j]));
+        } else {
+            out = support_analytic(h, dir);
+        }
+        out = apply_margin(h, out, dir);
+        a = vsel(k == 0, out, a);
+        b = vsel(k == 0, b, out);
+    }
+    return SupportPair{a, b};
+}
+
 // One work class, G lanes per pair, by one wavefront: groups of G lanes take
 // pairs of the class (chunked claims from the class cursor) until the class is
 // exhausted and every group of the wavefront has finished its pair.  The
-// wavefront's P/Q slots are `pq_wave` (24 x 64 doubles, slot stride 64 / G).
-template <int G, bool kHull>
+// wavefront's P/Q slots are `pq_wave` (24 x 64 doubles, slot stride 64 / G);
+// with kCache (hull classes, G >= 4) the rest of the region holds each slot's
+// vertex cache.
+template <int G, bool kHull, bool kCache = (kHull && G >= 4)>
 __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__restrict__ pa,
                                           const int32_t *__restrict__ pb,
                                           const int32_t *__restrict__ order, int begin, int count,
@@ -424,10 +516,16 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
     const int leader = lane - sub;
     const double tol2 = prm.tolerance * prm.tolerance;
     const PQ pq{pq_wave + lane / G, kWave / G, sub == 0};
+    float *const cache = reinterpret_cast<float *>(pq_wave + 24 * (kWave / G)) +
+                         (lane / G) * cache_stride_words<G>();
     GjkState g;
     gjk_init(g, pq);
     int ia = 0, ib = 0;
     int pid = 0;
+    // pairs with their hull vertices in the group's cache: hull vertices of
+    // A (the offset of B's), both records hulls
+    bool cached = false, two = false;
+    int nA = 0;
     bool have = false, exhausted = false;
     // The first chunk of every wavefront is fixed by its rank (wavefront index
     // minus `rot`, modulo the wavefront count): no atomic, so thousands of
@@ -485,6 +583,21 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                     gjk_init(g, pq);
                     if (ok) {
                         have = true;
+                        if constexpr (kCache) {
+                            const bool ha = A.type == D3D_HULL, hb = B.type == D3D_HULL;
+                            nA = ha ? A.nv : 0;
+                            const int nB = hb ? B.nv : 0;
+                            cached = nA + nB <= cache_verts<G>() && (!ha || A.voff64 < 0) &&
+                                     (!hb || B.voff64 < 0);
+                            two = ha && hb;
+                            if (cached) {
+                                if (ha) cache_fill<G>(s, A, cache, 0, sub);
+                                if (hb) cache_fill<G>(s, B, cache, nA, sub);
+                                // the group's lanes read what the others wrote
+                                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                            }
+                        }
                     } else if (sub == 0) {
                         gjk_write_bad(pid, o);
                     }
@@ -507,7 +620,15 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                 running = false;
             } else {
                 // the records were validated when the pair was taken
-                const SupportPair w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
+                SupportPair w;
+                if constexpr (kCache) {
+                    if (cached)
+                        w = support_pair_cached<G>(s, ia, ib, g.d, sub, cache, nA, two);
+                    else
+                        w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
+                } else {
+                    w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
+                }
                 running = gjk_distance_step(g, pq, w.a, w.b, prm.max_distance_squared, tol2);
             }
             if (!running) {
@@ -539,10 +660,10 @@ __global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
         const int begin = cls_begin[cls];
         int32_t *cur = cls_cursor + cls;
         const int G = prm.lanes_per_pair[cls];
-        // items per claim: 2 * (64 / G), or the class's fair share per
+        // items per claim: claim_factor * (64 / G), or the class's fair share per
         // wavefront when that is smaller, so a small batch spreads its pairs
         // over all wavefronts instead of a few wavefronts holding all of them
-        const int full = 2 * (kWave / G);
+        const int full = prm.claim_factor * (kWave / G);
         const int fair = (count + n_waves - 1) / n_waves;
         const int chunk = fair < full ? fair : full;
         if (cls == 0) {
@@ -625,11 +746,39 @@ __global__ void zero_u64_kernel(unsigned long long *c) {
     c[1] = 0ull;
 }
 
+// ---------------------------------------------------------- stage timing
+
+// Events of the calls recorded while stage timing is enabled: before the sort,
+// before the GJK kernel, after it, after EPA.  Spent event sets are reused.
+struct StageTiming {
+    std::mutex mu;
+    bool on = false;
+    std::vector<std::array<hipEvent_t, 4>> rec, pool;
+};
+static StageTiming g_timing;
+
+static bool stage_events(std::array<hipEvent_t, 4> &ev) {
+    std::lock_guard<std::mutex> lk(g_timing.mu);
+    if (!g_timing.on) return false;
+    if (!g_timing.pool.empty()) {
+        ev = g_timing.pool.back();
+        g_timing.pool.pop_back();
+    } else {
+        for (auto &e : ev)
+            if (hipEventCreate(&e) != hipSuccess) return false;
+    }
+    g_timing.rec.push_back(ev);
+    return true;
+}
+
 static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t n,
                       const d3d_gjk_params &prm, const GjkOut &o, void *ws, hipStream_t stream,
                       const EpaHook *epa) {
     const QueueWs w = ws_view(ws, n);
     const int hdr = (int)ws_header_ints();
+    std::array<hipEvent_t, 4> ev;
+    const bool timed = stage_events(ev);
+    if (timed) D3D_CHECK_HIP(hipEventRecord(ev[0], stream));
     hipLaunchKernelGGL(queue_clear_kernel, dim3(4), dim3(kQBlock), 0, stream, (int32_t *)ws, hdr);
     D3D_CHECK_LAUNCH("queue_clear_kernel");
     int64_t g = (n + kQBlock - 1) / kQBlock;
@@ -643,26 +792,36 @@ static int launch_all(const d3d_scene &s, const int32_t *pa, const int32_t *pb, 
     hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)g), dim3(kQBlock), 0, stream, s, pa, pb,
                        n, w.key_off, w.key_cur, w.order);
     D3D_CHECK_LAUNCH("queue_scatter_kernel");
-    // one persistent launch for every class; lanes resolved on the host
+    // one persistent launch for every class; lanes and launch shape resolved
+    // on the host
     d3d_gjk_params rp = prm;
     for (int c = 0; c < kNumClasses; ++c) {
         int G = prm.lanes_per_pair[c] > 0 ? prm.lanes_per_pair[c] : kDefaultLanes[c];
         if (c == 0 && G > 4) G = 4;
         rp.lanes_per_pair[c] = G;
     }
+    rp.blocks_per_cu = prm.blocks_per_cu > 0 ? prm.blocks_per_cu : 3;
+    rp.claim_factor = prm.claim_factor > 0 ? prm.claim_factor : 2;
+    rp.epa_blocks_per_cu = prm.epa_blocks_per_cu > 0 ? prm.epa_blocks_per_cu : 8;
     if (epa) {
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, stream, epa->count);
         D3D_CHECK_LAUNCH("zero_u64_kernel");
     }
-    // resident at occupancy 3 (4 wavefronts per block)
-    const int64_t grid = (int64_t)cu_count() * 3;
+    // resident at occupancy 3 (4 wavefronts per block) by default
+    const int64_t grid = (int64_t)cu_count() * rp.blocks_per_cu;
+    if (timed) D3D_CHECK_HIP(hipEventRecord(ev[1], stream));
     hipLaunchKernelGGL(gjk_all_kernel, dim3((unsigned)grid), dim3(kQBlock), 0, stream, s, pa, pb,
                        w.order, w.cls_begin, w.cls_count, w.cls_cursor, rp, o);
     D3D_CHECK_LAUNCH("gjk_all_kernel");
-    if (epa)
-        return launch_epa(s, pa, pb, n, (const int64_t *)epa->count, epa->work, nullptr, o.simplex,
-                          o.npts, epa->max_iter, epa->max_faces, epa->eps, epa->out, epa->status,
-                          nullptr, epa->count + 1, stream);
+    if (timed) D3D_CHECK_HIP(hipEventRecord(ev[2], stream));
+    if (epa) {
+        if (int rc = launch_epa(s, pa, pb, n, (const int64_t *)epa->count, epa->work, nullptr,
+                                o.simplex, o.npts, epa->max_iter, epa->max_faces, epa->eps,
+                                epa->out, epa->status, nullptr, epa->count + 1, stream, nullptr,
+                                nullptr, rp.epa_blocks_per_cu))
+            return rc;
+    }
+    if (timed) D3D_CHECK_HIP(hipEventRecord(ev[3], stream));
     return D3D_OK;
 }
 
@@ -720,6 +879,42 @@ __global__ __launch_bounds__(kQBlock) void iota_pairs_kernel(int32_t *__restrict
 
 using namespace d3d;
 
+extern "C" int d3d_stage_timing(int enable) {
+    clear_error();
+    std::lock_guard<std::mutex> lk(g_timing.mu);
+    g_timing.on = enable != 0;
+    // events of calls not yet read are dropped unread (they may still be in
+    // flight: destroyed only once they have completed)
+    for (auto &ev : g_timing.rec) {
+        (void)hipEventSynchronize(ev[3]);
+        g_timing.pool.push_back(ev);
+    }
+    g_timing.rec.clear();
+    return D3D_OK;
+}
+
+extern "C" int d3d_stage_timing_read(double *ms_out, int64_t *calls_out) {
+    clear_error();
+    D3D_CHECK_ARG(ms_out && calls_out, "NULL buffer");
+    std::lock_guard<std::mutex> lk(g_timing.mu);
+    double sum[3] = {0.0, 0.0, 0.0};
+    int64_t calls = 0;
+    for (auto &ev : g_timing.rec) {
+        D3D_CHECK_HIP(hipEventSynchronize(ev[3]));
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.0f;
+            D3D_CHECK_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            sum[k] += ms;
+        }
+        ++calls;
+        g_timing.pool.push_back(ev);
+    }
+    g_timing.rec.clear();
+    for (int k = 0; k < 3; ++k) ms_out[k] = sum[k];
+    *calls_out = calls;
+    return D3D_OK;
+}
+
 extern "C" int64_t d3d_gjk_workspace_bytes(int64_t n) {
     if (n < 0) return -1;
     return (ws_header_ints() + n) * (int64_t)sizeof(int32_t);
@@ -747,8 +942,13 @@ static int gjk_distance_impl(const d3d_scene *scene, const int32_t *pair_a, cons
         prm.max_iterations = 128;
         prm.hull_serial_max = 16;
         for (int c = 0; c < kNumClasses; ++c) prm.lanes_per_pair[c] = 0;
+        prm.blocks_per_cu = prm.claim_factor = prm.epa_blocks_per_cu = 0;
     }
     D3D_CHECK_ARG(prm.max_iterations > 0, "max_iterations must be > 0");
+    D3D_CHECK_ARG(prm.blocks_per_cu >= 0 && prm.blocks_per_cu <= 3, "blocks_per_cu must be in [0, 3]");
+    D3D_CHECK_ARG(prm.claim_factor >= 0 && prm.claim_factor <= 8, "claim_factor must be in [0, 8]");
+    D3D_CHECK_ARG(prm.epa_blocks_per_cu >= 0 && prm.epa_blocks_per_cu <= 16,
+                  "epa_blocks_per_cu must be in [0, 16]");
     for (int c = 0; c < kNumClasses; ++c) {
         const int G = prm.lanes_per_pair[c];
         D3D_CHECK_ARG(G == 0 || (G <= 64 && (G & (G - 1)) == 0),

@@ -78,7 +78,7 @@ int launch_epa(const d3d_scene &s, const int32_t *pa, const int32_t *pb, int64_t
                const double *simplex, const int32_t *npts, int max_iter, int max_faces,
                double eps, double *out, int32_t *status, int32_t *iters,
                unsigned long long *next, hipStream_t stream, double *faces = nullptr,
-               int32_t *n_faces = nullptr);
+               int32_t *n_faces = nullptr, int blocks_per_cu = 8);
 
 // Device address of d3d_gjk_epa_batch_noalloc's scratch array on the current
 // device (looked up once per device with hipGetSymbolAddress); out may be NULL.

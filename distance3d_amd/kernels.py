@@ -67,14 +67,14 @@ class GjkResult:
 def gjk_distance_batch(scene, pair_a, pair_b, tolerance=1e-10, max_distance_squared=100000.0,
                        sanity_check=1e-8, max_iterations=128, lanes_per_pair=None,
                        want_simplex=False, want_iterations=True, out=None, workspace=None,
-                       stream=None):
+                       stream=None, launch=None):
     """GJK distance over pairs (gjk.gjk semantics per pair).
 
     ``out`` may hold preallocated tensors (dist, cp_a, cp_b, status,
     iterations, simplex, n_points) and ``workspace`` a uint8 device tensor of
     at least :func:`gjk_workspace_bytes` bytes, so a loop allocates nothing.
-    ``lanes_per_pair`` (8 entries, one per work class) overrides the launch
-    tuning."""
+    ``lanes_per_pair`` (8 entries, one per work class) and ``launch`` (dict:
+    ``blocks_per_cu``, ``claim_factor``) override the launch tuning."""
     import torch
     a = _dev_i32(pair_a, scene.device)
     b = _dev_i32(pair_b, scene.device)
@@ -92,7 +92,7 @@ def gjk_distance_batch(scene, pair_a, pair_b, tolerance=1e-10, max_distance_squa
             torch.empty((n, 3, 4, 3), dtype=torch.float64, device=dev) if want_simplex else None,
             torch.empty(n, dtype=torch.int32, device=dev) if want_simplex else None)
     prm = _native.gjk_params(tolerance, max_distance_squared, sanity_check, max_iterations,
-                             lanes_per_pair=lanes_per_pair)
+                             lanes_per_pair=lanes_per_pair, launch=launch)
     import ctypes
     if workspace is None:
         check(lib().d3d_gjk_distance_batch(

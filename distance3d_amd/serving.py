@@ -135,7 +135,7 @@ class GpuSlot:
         res = kernels.GjkEpaResult(o.dist[:n], o.cp_a[:n], o.cp_b[:n], o.status[:n],
                                    o.epa_out[:n], o.epa_status[:n])
         kernels.gjk_epa_batch(view, pa, pb, out=res, workspace=self.ws,
-                              lanes_per_pair=srv.lanes_per_pair)
+                              lanes_per_pair=srv.lanes_per_pair, launch=srv.launch)
         self.h_dist[:n].copy_(o.dist[:n], non_blocking=True)
         self.h_cp[0, :n].copy_(o.cp_a[:n], non_blocking=True)
         self.h_cp[1, :n].copy_(o.cp_b[:n], non_blocking=True)
@@ -223,12 +223,13 @@ class CollisionServer:
     """Continuous batching over a device-resident scene and pose table."""
 
     def __init__(self, scene, poses, n_slots=4, slot_pairs=16384, lanes_per_pair=None,
-                 use_graphs=True, slot_factory=None):
+                 use_graphs=True, slot_factory=None, launch=None):
         """``scene``: DeviceScene; ``poses``: (P, 12) float64 pose rows (R row
         major, t), kept on the device with two extra rows used to pad
         batches (a pair placed 1e4 apart, clipped in one GJK iteration).
         ``lanes_per_pair``: per work class; None takes the autotuner's
-        latency table."""
+        latency table.  ``launch``: launch shape (see kernels.gjk_epa_batch);
+        None takes the autotuner's latency table, else the library default."""
         import torch
         self.scene = scene
         P = np.asarray(poses, dtype=np.float64).reshape(-1, 12)
@@ -246,7 +247,11 @@ class CollisionServer:
             # time of one small batch), else the library defaults
             from . import autotune
             lanes_per_pair = autotune.tuned_lanes(kind="latency")
+        if launch is None:
+            from . import autotune
+            launch = autotune.tuned_launch(kind="latency")
         self.lanes_per_pair = lanes_per_pair
+        self.launch = launch
         queues = os.environ.get("GPU_MAX_HW_QUEUES")
         if slot_factory is None and (queues is None or int(queues) < n_slots + 2):
             warnings.warn("GPU_MAX_HW_QUEUES=%s: fewer hardware queues than the %d slot streams "

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define D3D_ABI_VERSION 3
+#define D3D_ABI_VERSION 4
 
 /* error codes */
 #define D3D_OK 0
@@ -111,8 +111,16 @@ typedef struct d3d_scene {
  *   5: <= 2048, 6: <= 4096, 7: > 4096.
  * lanes_per_pair[c] lanes (a power of two <= 64; 0 = built-in default
  * 1,1,1,1,2,4,8,16) work on each pair of class c (class 0: 1, 2 or 4; larger
- * values use 4).  With 2..32 lanes the two halves of a group evaluate the
- * supports on A and on B concurrently. */
+ * values use 4).  For hull classes with 4 or more lanes, a pair whose hull
+ * vertices fit the group's LDS slot (44, 109, 240, 480 or 1008 vertices with
+ * 4, 8, 16, 32 or 64 lanes) has them copied there once and is swept from LDS;
+ * when both colliders are hulls the two halves of the group sweep A and B
+ * concurrently.
+ * Launch shape (0 = built-in default): blocks_per_cu persistent GJK blocks of
+ * four wavefronts per compute unit (1..3, default 3); claim_factor: a
+ * wavefront claims claim_factor * 64 / G pairs of a class at a time (1..8,
+ * default 2; fewer when the class is small); epa_blocks_per_cu persistent EPA
+ * wavefronts per compute unit in the fused pipeline (1..16, default 8). */
 typedef struct d3d_gjk_params {
     double tolerance;            /* 1e-10 */
     double max_distance_squared; /* 1e5 */
@@ -122,6 +130,9 @@ typedef struct d3d_gjk_params {
                                     <= this many vertices are swept by their
                                     own lane, larger ones by the wavefront */
     int32_t lanes_per_pair[8];
+    int32_t blocks_per_cu;
+    int32_t claim_factor;
+    int32_t epa_blocks_per_cu;
 } d3d_gjk_params;
 
 /* Library version / device setup. */
@@ -189,6 +200,21 @@ int d3d_gjk_distance_batch(const d3d_scene *scene, const int32_t *pair_a, const 
                            int64_t n, const d3d_gjk_params *params, double *dist, double *cp_a,
                            double *cp_b, int32_t *status, int32_t *iterations, double *simplex,
                            int32_t *n_points, void *stream);
+
+/*
+ * Stage timing of the GJK pipelines (d3d_gjk_distance_batch*,
+ * d3d_gjk_epa_batch*); no counterpart in the reference.  While enabled, every
+ * call records HIP events on its stream before the counting sort, before the
+ * GJK kernel, after it and after the EPA kernel.  d3d_stage_timing(1) starts
+ * (dropping what was recorded), d3d_stage_timing(0) stops.
+ * d3d_stage_timing_read waits for the recorded events, writes the summed
+ * milliseconds of the three stages (counting sort, GJK kernel, EPA kernel; 0
+ * without EPA) to ms_out[3] and the number of calls to calls_out, and forgets
+ * them.  Meant for measurement: calls under graph capture must not run with it
+ * enabled.
+ */
+int d3d_stage_timing(int enable);
+int d3d_stage_timing_read(double *ms_out, int64_t *calls_out);
 
 /* Bytes of device workspace d3d_gjk_distance_batch_ws needs for n pairs. */
 int64_t d3d_gjk_workspace_bytes(int64_t n);
