@@ -164,6 +164,14 @@ __device__ __forceinline__ bool face_normal(v3 w0, v3 w1, v3 w2, v3 &n, double &
     return true;
 }
 
+// face_normal's verdict alone: sqrt(x) > 0 exactly when x > 0 (x = c . c is
+// never negative; NaN fails both), so no square root or division is needed to
+// know whether the face is kept.
+__device__ __forceinline__ bool face_valid(v3 w0, v3 w1, v3 w2) {
+    const v3 c = cross(w1 - w0, w2 - w0);
+    return dot(c, c) > 0.0;
+}
+
 // Polytope vertex row r of the LDS arrays: Minkowski point w and the support
 // points on A and B.  Every lane of the wavefront computes the same values and
 // writes them to the same addresses, so rows need no broadcast.
@@ -531,9 +539,7 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                     const int a = fv[sl][e], b = fv[sl][(e + 1) % 3];
                     if (edge_has(L, b, a)) continue;
                     hzm |= 1 << (3 * sl + e);
-                    v3 nn;
-                    double dd;
-                    if (face_normal(lds_w(L, a), lds_w(L, b), w, nn, dd)) {
+                    if (face_valid(lds_w(L, a), lds_w(L, b), w)) {
                         hvm |= 1 << (3 * sl + e);
                         ++cnt[sl];
                     }
@@ -590,22 +596,31 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
                 for (int e = 0; e < 3; ++e) {
                     if (!((hvm >> (3 * sl + e)) & 1)) continue;
                     if (pos < max_faces) {
-                        // same expression as in the scan above: same bits
-                        const int a = fv[sl][e], b = fv[sl][(e + 1) % 3];
-                        v3 nn;
-                        double dd;
-                        face_normal(lds_w(L, a), lds_w(L, b), w, nn, dd);
-                        L.fv0[pos] = a;
-                        L.fv1[pos] = b;
+                        L.fv0[pos] = fv[sl][e];
+                        L.fv1[pos] = fv[sl][(e + 1) % 3];
                         L.fv2[pos] = wi;
-                        L.fnx[pos] = nn.x; L.fny[pos] = nn.y; L.fnz[pos] = nn.z;
-                        L.fd[pos] = dd;
                     }
                     ++pos;
                 }
             }
             const int added = tot0 + tot1;
             nf = nkept + (added < max_faces - nkept ? added : (max_faces - nkept > 0 ? max_faces - nkept : 0));
+            __syncthreads();
+            // the new faces' unit normals and distances, one face per lane
+            // (face_normal of the vertex rows the indices name: the same
+            // operands as the verdict above, so the same bits as computing
+            // them there)
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int k = nkept + sl * kWave + lane;
+                if (k < nf) {
+                    v3 nn = mk(0.0, 0.0, 0.0);
+                    double dd = 0.0;
+                    face_normal(lds_w(L, L.fv0[k]), lds_w(L, L.fv1[k]), lds_w(L, L.fv2[k]), nn, dd);
+                    L.fnx[k] = nn.x; L.fny[k] = nn.y; L.fnz[k] = nn.z;
+                    L.fd[k] = dd;
+                }
+            }
             __syncthreads();
             if (cap || nf == 0) {
                 status = 2;
