@@ -425,9 +425,9 @@ __device__ __forceinline__ void rescan_take(const d3d_scene &s, const HullRef &H
 // every vertex tied with it score at least m1 - E - 2d exactly (d: FP64
 // rounding, far below E), hence at least m1 - 2E - 2d in FP32, so no
 // candidate is missed.  A NaN or -inf threshold evaluates every vertex.
-template <int G>
+template <int G, bool kBounds = false>
 __device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H, v3 l, int sub,
-                                           float thr) {
+                                           float thr, const float4 *bounds = nullptr) {
     constexpr int K = D3D_CLUSTER_SIZE;
     const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
     const float *__restrict__ px = s.vx + H.voff;
@@ -444,7 +444,8 @@ __device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H,
         for (int j = sub; j < nv; j += G)
             if (!(score(j) < thr)) rescan_take(s, H, l, j, A);
     } else {
-        const float4 *__restrict__ cl = reinterpret_cast<const float4 *>(s.clusters) + H.coff;
+        const float4 *__restrict__ cl =
+            kBounds ? bounds : reinterpret_cast<const float4 *>(s.clusters) + H.coff;
         const float nl = (float)(sqrt(l.x * l.x + l.y * l.y + l.z * l.z) * (1.0 + 0x1p-19));
         for (int k = sub; k < H.nc; k += G) {
             const float4 c = cl[k];
@@ -474,8 +475,11 @@ __device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H,
     return A.group_result<G>();
 }
 
-template <int G>
-__device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H, v3 l, int sub) {
+// With kBounds, `bounds` is a copy of the hull's cluster records (in LDS)
+// read instead of the scene's; the result is the same.
+template <int G, bool kBounds = false>
+__device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H, v3 l, int sub,
+                                           const float4 *bounds = nullptr) {
     constexpr int K = D3D_CLUSTER_SIZE;
     const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
     const float *__restrict__ px = s.vx + H.voff;
@@ -501,7 +505,8 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
         }
         for (; k < nv; k += G) top2(score(k), k, m1, m2, i1);
     } else {
-        const float4 *__restrict__ cl = reinterpret_cast<const float4 *>(s.clusters) + H.coff;
+        const float4 *__restrict__ cl =
+            kBounds ? bounds : reinterpret_cast<const float4 *>(s.clusters) + H.coff;
         const int nc = H.nc;
         const float nl = (float)(sqrt(l.x * l.x + l.y * l.y + l.z * l.z) * (1.0 + 0x1p-19));
         auto bound = [&](int k) {
@@ -558,7 +563,7 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
     group_top2<G>(m1, m2, i1);
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
 
-    return hull_rescan<G>(s, H, l, sub, __double2float_rd((double)m1 - 2.5 * E));
+    return hull_rescan<G, kBounds>(s, H, l, sub, __double2float_rd((double)m1 - 2.5 * E), bounds);
 }
 
 // Argmax over a hull whose FP32 vertices sit in LDS in their ORIGINAL order

@@ -499,6 +499,48 @@ __device__ __forceinline__ SupportPair support_pair_cached(const d3d_scene &s, i
     return SupportPair{a, b};
 }
 
+// Copies the cluster records of hull record h (validated, clustered) into
+// the slot at record `at`, by the G lanes of a group.
+template <int G>
+__device__ __forceinline__ void bounds_fill(const d3d_scene &s, const Shape &h, float4 *slot, int at,
+                                            int sub) {
+    const float4 *__restrict__ cl = reinterpret_cast<const float4 *>(s.clusters) + (int64_t)h.p[1];
+    const int nc = (int)h.p[2];
+#pragma unroll 4
+    for (int k = sub; k < nc; k += G) slot[at + k] = cl[k];
+}
+
+// Supports of records ia along d and ib along -d (see support_pair) for a pair
+// whose clustered hulls have their cluster records in the group's slot (A's
+// first, ncA of them): the sweeps read the bounds from LDS and the vertices
+// from the pools.  Same results as support_pair.
+template <int G>
+__device__ __forceinline__ SupportPair support_pair_bounds(const d3d_scene &s, int ia, int ib, v3 d,
+                                                           int sub, const float4 *slot, int ncA) {
+    v3 a = mk(0.0, 0.0, 0.0), b = a;
+    if (G == kWave) d = mk(wave_uniform(d.x), wave_uniform(d.y), wave_uniform(d.z));
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const int idx = k == 0 ? ia : ib;
+        const Shape h = load_shape(s, G == kWave ? wave_uniform(idx) : opaque(idx));
+        const v3 dir = vsel(k == 0, d, -d);
+        v3 out;
+        if (h.type == D3D_HULL) {
+            const v3 l = to_local(h, dir);
+            const HullRef H = hull_ref(h);
+            const int hi = H.nc > 0 ? hull_argmax<G, true>(s, H, l, sub, slot + (k == 0 ? 0 : ncA))
+                                    : hull_argmax<G>(s, H, l, sub);
+            out = to_world(h, hull_vertex(s, h.voff, h.voff64, hi));
+        } else {
+            out = support_analytic(h, dir);
+        }
+        out = apply_margin(h, out, dir);
+        a = vsel(k == 0, out, a);
+        b = vsel(k == 0, b, out);
+    }
+    return SupportPair{a, b};
+}
+
 // One work class, G lanes per pair, by one wavefront: groups of G lanes take
 // pairs of the class (chunked claims from the class cursor) until the class is
 // exhausted and every group of the wavefront has finished its pair.  The
@@ -526,6 +568,10 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
     // A (the offset of B's), both records hulls
     bool cached = false, two = false;
     int nA = 0;
+    // otherwise, pairs with their hulls' cluster records in the slot (ncA: A's
+    // record count, the offset of B's)
+    bool bounded = false;
+    int ncA = 0;
     bool have = false, exhausted = false;
     // The first chunk of every wavefront is fixed by its rank (wavefront index
     // minus `rot`, modulo the wavefront count): no atomic, so thousands of
@@ -590,13 +636,21 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                             cached = nA + nB <= cache_verts<G>() && (!ha || A.voff64 < 0) &&
                                      (!hb || B.voff64 < 0);
                             two = ha && hb;
+                            ncA = ha ? (int)A.p[2] : 0;
+                            const int ncB = hb ? (int)B.p[2] : 0;
+                            bounded = !cached && ncA + ncB > 0 &&
+                                      4 * (ncA + ncB) <= cache_stride_words<G>();
                             if (cached) {
                                 if (ha) cache_fill<G>(s, A, cache, 0, sub);
                                 if (hb) cache_fill<G>(s, B, cache, nA, sub);
-                                // the group's lanes read what the others wrote
-                                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                            } else if (bounded) {
+                                float4 *slot = reinterpret_cast<float4 *>(cache);
+                                if (ncA > 0) bounds_fill<G>(s, A, slot, 0, sub);
+                                if (ncB > 0) bounds_fill<G>(s, B, slot, ncA, sub);
                             }
+                            // the group's lanes read what the others wrote
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                         }
                     } else if (sub == 0) {
                         gjk_write_bad(pid, o);
@@ -624,6 +678,9 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                 if constexpr (kCache) {
                     if (cached)
                         w = support_pair_cached<G>(s, ia, ib, g.d, sub, cache, nA, two);
+                    else if (bounded)
+                        w = support_pair_bounds<G>(s, ia, ib, g.d, sub,
+                                                   reinterpret_cast<const float4 *>(cache), ncA);
                     else
                         w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
                 } else {
