@@ -675,7 +675,8 @@ def run_contact(args, steps=10, warmup=2):
 
 def class_breakdown(w, lanes, reps=5, launch=None):
     """Each work class of the workload (pairs sorted by their total hull
-    vertex count, 8 classes) as a GJK batch of its own: event time, pairs,
+    vertex count and whether both colliders are hulls, 16 classes) as a GJK
+    batch of its own: event time, pairs,
     GJK iterations, and the algorithmic bytes 12 (n_A + n_B) + 196 per pair
     over that time.  Each class is one set of dispatches, so a kernel trace
     of this mode gives per-class kernel times and counters."""
@@ -685,7 +686,7 @@ def class_breakdown(w, lanes, reps=5, launch=None):
     cls = autotune.pair_classes(w.scene, w.pair_a, w.pair_b)
     hb = w.hull_bytes_per_pair()
     rows = {}
-    for c in range(8):
+    for c in range(16):
         idx = np.nonzero(cls == c)[0]
         if len(idx) == 0:
             continue

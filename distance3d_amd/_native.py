@@ -47,7 +47,7 @@ class d3d_scene(ctypes.Structure):
     ]
 
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 CLUSTER_SIZE = 16
 NOALLOC_MAX_PAIRS = 16384  # D3D_NOALLOC_MAX_PAIRS
 
@@ -59,7 +59,7 @@ class d3d_gjk_params(ctypes.Structure):
         ("sanity_check", ctypes.c_double),
         ("max_iterations", ctypes.c_int32),
         ("hull_serial_max", ctypes.c_int32),
-        ("lanes_per_pair", ctypes.c_int32 * 8),
+        ("lanes_per_pair", ctypes.c_int32 * 16),
         ("blocks_per_cu", ctypes.c_int32),
         ("claim_factor", ctypes.c_int32),
         ("epa_blocks_per_cu", ctypes.c_int32),
@@ -180,7 +180,22 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-DEFAULT_LANES = (1, 1, 1, 1, 2, 4, 8, 16)
+# work classes: size class 0..7 (total hull vertices 0, <=32, <=128, <=512,
+# <=1024, <=2048, <=4096, more), plus 8 when both colliders are hulls
+NUM_CLASSES = 16
+DEFAULT_LANES = (1, 4, 16, 8, 8, 16, 64, 64, 1, 8, 16, 16, 64, 16, 64, 64)
+
+
+def expand_lanes(lanes):
+    """A lanes table for the 16 work classes: 16 entries as given, or 8 (one
+    per size class) used for pairs of one hull and of two hulls alike."""
+    lanes = [int(g) for g in lanes]
+    if len(lanes) == NUM_CLASSES // 2:
+        lanes = lanes * 2
+    if len(lanes) != NUM_CLASSES:
+        raise ValueError("lanes_per_pair needs 16 entries (one per work class) or 8 (one per "
+                         "size class)")
+    return lanes
 
 
 def gjk_params(tolerance=1e-10, max_distance_squared=100000.0, sanity_check=1e-8,
@@ -194,9 +209,7 @@ def gjk_params(tolerance=1e-10, max_distance_squared=100000.0, sanity_check=1e-8
     p.sanity_check = float(sanity_check)
     p.max_iterations = int(max_iterations)
     p.hull_serial_max = int(hull_serial_max)
-    lanes = DEFAULT_LANES if lanes_per_pair is None else lanes_per_pair
-    if len(lanes) != 8:
-        raise ValueError("lanes_per_pair needs 8 entries (one per work class)")
+    lanes = expand_lanes(DEFAULT_LANES if lanes_per_pair is None else lanes_per_pair)
     for c, g in enumerate(lanes):
         p.lanes_per_pair[c] = int(g)
     launch = launch or {}

@@ -2,9 +2,10 @@
 the launch shape (the tile of pairs a wavefront claims, blocks per compute
 unit).
 
-The GJK pipeline sorts pairs into 8 classes by their total hull vertex count
-(0, <=32, <=128, <=512, <=1024, <=2048, <=4096, >4096).  The tables kept in
-``tuning.json`` beside this file:
+The GJK pipeline sorts pairs into 16 work classes: 8 size classes by their
+total hull vertex count (0, <=32, <=128, <=512, <=1024, <=2048, <=4096,
+>4096), each split into pairs with one hull and pairs of two hulls (class + 8).
+The tables kept in ``tuning.json`` beside this file:
 
 * ``lanes_per_pair`` (throughput, :func:`tune`): for each class, the GJK
   kernels are timed on that class's own pairs (taken from a large sample
@@ -37,10 +38,14 @@ from . import _native
 
 TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 CLASS_EDGES = (0, 32, 128, 512, 1024, 2048, 4096)
-CANDIDATES = {0: (1, 2, 4), 1: (1, 2, 4, 8), 2: (2, 4, 8, 16), 3: (4, 8, 16, 32),
-              4: (8, 16, 32, 64), 5: (8, 16, 32, 64), 6: (16, 32, 64), 7: (16, 32, 64)}
-LATENCY_CANDIDATES = {0: (1, 2, 4), 1: (4, 8, 16, 32), 2: (8, 16, 32, 64), 3: (16, 32, 64),
-                      4: (16, 32, 64), 5: (32, 64), 6: (64,), 7: (64,)}
+NUM_CLASSES = _native.NUM_CLASSES
+_SIZE_CANDIDATES = {0: (1, 2, 4), 1: (1, 2, 4, 8), 2: (2, 4, 8, 16), 3: (4, 8, 16, 32),
+                    4: (8, 16, 32, 64), 5: (8, 16, 32, 64), 6: (16, 32, 64), 7: (16, 32, 64)}
+_SIZE_LATENCY = {0: (1, 2, 4), 1: (4, 8, 16, 32), 2: (8, 16, 32, 64), 3: (16, 32, 64),
+                 4: (16, 32, 64), 5: (32, 64), 6: (64,), 7: (64,)}
+# the same candidates for pairs of one hull (class c) and of two (c + 8)
+CANDIDATES = {c: _SIZE_CANDIDATES[c % 8] for c in range(NUM_CLASSES)}
+LATENCY_CANDIDATES = {c: _SIZE_LATENCY[c % 8] for c in range(NUM_CLASSES)}
 
 
 LAUNCH_CANDIDATES = {"blocks_per_cu": (2, 3), "claim_factor": (1, 2, 4),
@@ -81,7 +86,7 @@ def descend(measure, start, candidates, sweeps=1, log=None):
 
 
 def work_class(nverts):
-    """Host restatement of the device work-class rule (d3d.h)."""
+    """Host restatement of the device size-class rule (d3d.h)."""
     nverts = np.asarray(nverts)
     cls = np.full(nverts.shape, 7, dtype=np.int32)
     for c in range(6, -1, -1):
@@ -91,9 +96,11 @@ def work_class(nverts):
 
 
 def pair_classes(scene, pair_a, pair_b):
+    """Work class of each pair: size class, + 8 when both colliders are hulls."""
     i = scene.shape_i32
     nv = np.where(i[:, 0] == _native.HULL, i[:, 1], 0).astype(np.int64)
-    return work_class(nv[pair_a] + nv[pair_b])
+    na, nb = nv[pair_a], nv[pair_b]
+    return work_class(na + nb) + np.where((na > 0) & (nb > 0), 8, 0).astype(np.int32)
 
 
 def tuned_lanes(path=TUNING_PATH, kind="throughput"):
@@ -104,7 +111,7 @@ def tuned_lanes(path=TUNING_PATH, kind="throughput"):
     with open(path) as f:
         t = json.load(f)
     lanes = t.get("lanes_per_pair" if kind == "throughput" else "serving_lanes_per_pair")
-    return None if lanes is None else [int(x) for x in lanes]
+    return None if lanes is None else _native.expand_lanes(lanes)
 
 
 def tuned_launch(path=TUNING_PATH, kind="throughput"):
@@ -143,7 +150,7 @@ def tune(workload, reps=3, log=print):
     cls = pair_classes(workload.scene, workload.pair_a, workload.pair_b)
     lanes = list(_native.DEFAULT_LANES)
     report = {}
-    for c in range(0, 8):
+    for c in range(NUM_CLASSES):
         idx = np.nonzero(cls == c)[0]
         if len(idx) < 256:
             continue
@@ -159,6 +166,19 @@ def tune(workload, reps=3, log=print):
         report[c] = {"pairs": int(len(idx)), "ms": {str(k): round(v * 1e3, 4) for k, v in
                                                     times.items()}, "best": best}
         log("class %d (%d pairs): %s -> %d lanes" % (c, len(idx), report[c]["ms"], best))
+    # classes the workload leaves (nearly) empty take the choice of the same
+    # size with the other hull count, else of the next smaller tuned size
+    tuned = set(report)
+    for c in range(NUM_CLASSES):
+        if c in tuned or c % 8 == 0:
+            continue
+        other = (c + 8) % NUM_CLASSES
+        if other in tuned:
+            lanes[c] = lanes[other]
+        else:
+            smaller = [k for k in tuned if k // 8 == c // 8 and k % 8 < c % 8]
+            if smaller:
+                lanes[c] = lanes[max(smaller)]
     return lanes, report
 
 
@@ -240,11 +260,13 @@ def tune_latency(scene, poses, requests, start=None, sweeps=2, reps=3, log=print
             best.append(min(ts))
         return float(np.median(best))
 
-    lanes = list(start or _native.DEFAULT_LANES)
+    lanes = _native.expand_lanes(start or _native.DEFAULT_LANES)
     measure(lanes)  # warm-up
     report = {"start": {"lanes": list(lanes), "ms": round(1e3 * measure(lanes), 4)}}
     for sweep in range(sweeps):
-        for c in range(8):
+        for c in range(NUM_CLASSES):
+            if c == 8:
+                continue  # two hulls always hold vertices
             times = {}
             for g in LATENCY_CANDIDATES[c]:
                 trial = list(lanes)

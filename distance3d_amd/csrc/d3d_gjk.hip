@@ -266,7 +266,7 @@ static d3d_gjk_params default_params() {
     p.sanity_check = 1e-8;
     p.max_iterations = 128;
     p.hull_serial_max = 16;
-    for (int c = 0; c < 8; ++c) p.lanes_per_pair[c] = 0;
+    for (int c = 0; c < D3D_NUM_CLASSES; ++c) p.lanes_per_pair[c] = 0;
     p.blocks_per_cu = 0;
     p.claim_factor = 0;
     p.epa_blocks_per_cu = 0;

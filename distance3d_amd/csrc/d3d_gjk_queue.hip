@@ -27,12 +27,14 @@
 namespace d3d {
 
 constexpr int kQBlock = 256;
-constexpr int kNumClasses = 8;
+constexpr int kSizeClasses = 8;
+constexpr int kNumClasses = D3D_NUM_CLASSES;  // size class, + 8 when both are hulls
+static_assert(kNumClasses == 2 * kSizeClasses, "work classes: size x one or two hulls");
 constexpr int kNumTypes = 9;
 constexpr int kNumKeys = kNumClasses * kNumTypes * kNumTypes;
 constexpr int kBadInput = 0x200;
 
-// work class from the total hull vertex count of a pair
+// size class from the total hull vertex count of a pair
 __device__ __forceinline__ int work_class(int nverts) {
     if (nverts == 0) return 0;
     if (nverts <= 32) return 1;
@@ -52,7 +54,7 @@ __device__ __forceinline__ int pair_key(const d3d_scene &s, int a, int b) {
     const int tb = (B.x >= 0 && B.x < kNumTypes) ? B.x : 0;
     const int na = (ta == D3D_HULL) ? (A.y > 0 ? A.y : 0) : 0;
     const int nb = (tb == D3D_HULL) ? (B.y > 0 ? B.y : 0) : 0;
-    const int cls = work_class(na + nb);
+    const int cls = work_class(na + nb) + (na > 0 && nb > 0 ? kSizeClasses : 0);
     return (cls * kNumTypes + ta) * kNumTypes + tb;
 }
 
@@ -697,8 +699,9 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
 }
 
 // All work classes in one persistent launch: every wavefront walks the classes
-// from the largest hulls (7) to the analytic pairs (0), switching its lanes per
-// pair at each class, so classes overlap without a launch per class.
+// from the largest hulls to the analytic pairs (size classes 7 .. 0, the pairs
+// of two hulls before those of one at each size), switching its lanes per pair
+// at each class, so classes overlap without a launch per class.
 // lanes_per_pair holds resolved values (1..64, class 0: 1, 2 or 4).
 // Occupancy 3 (<= 168 VGPRs): measured faster than occupancy 2 without the
 // register cap, at the cost of a few spilled registers.
@@ -711,7 +714,8 @@ __global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
     double *pq_wave = pq_lds + 24 * kWave * (threadIdx.x / kWave);
     const int n_waves = (int)gridDim.x * (kQBlock / kWave);
     int rot = 0;  // wavefronts given a fixed chunk by the classes walked so far
-    for (int cls = kNumClasses - 1; cls >= 0; --cls) {
+    for (int step = 0; step < kNumClasses; ++step) {
+        const int cls = (step & 1) ? kSizeClasses - 1 - step / 2 : kNumClasses - 1 - step / 2;
         const int count = cls_count[cls];
         if (count == 0) continue;
         const int begin = cls_begin[cls];
@@ -745,7 +749,9 @@ __global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
 
 // ---------------------------------------------------------------- host
 
-static const int kDefaultLanes[kNumClasses] = {1, 1, 1, 1, 2, 4, 8, 16};
+// the autotuner's table for the 1,048,576-pair mixed workload on one MI355X
+static const int kDefaultLanes[kNumClasses] = {1, 4, 16, 8, 8, 16, 64, 64,
+                                               1, 8, 16, 16, 64, 16, 64, 64};
 
 struct WsCache {
     std::mutex mu;

@@ -20,24 +20,13 @@ constexpr int kTetSame = 0x10;
 constexpr int kTetCuts = 0x20;  // the plane cuts both tetrahedra
 constexpr int kTetOverflow = 0x100;
 
-// numpy's pairwise sum (n <= 128) of a[0], a[st], ...: a plain loop below 8
-// terms, else eight running sums combined pairwise, then the tail
-__device__ __forceinline__ double np_sum(const double *a, int n, int st) {
-    if (n < 8) {
-        double r = 0.0;
-        for (int i = 0; i < n; ++i) r += a[i * st];
-        return r;
-    }
-    double r0 = a[0], r1 = a[st], r2 = a[2 * st], r3 = a[3 * st];
-    double r4 = a[4 * st], r5 = a[5 * st], r6 = a[6 * st], r7 = a[7 * st];
-    int i = 8;
-    for (; i < n - (n % 8); i += 8) {
-        r0 += a[i * st]; r1 += a[(i + 1) * st]; r2 += a[(i + 2) * st]; r3 += a[(i + 3) * st];
-        r4 += a[(i + 4) * st]; r5 += a[(i + 5) * st]; r6 += a[(i + 6) * st]; r7 += a[(i + 7) * st];
-    }
-    double res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
-    for (; i < n; ++i) res += a[i * st];
-    return res;
+// np.mean of a[0], a[st], ... as order_points computes it: the function is
+// compiled by numba (_tetrahedron_intersection.py:299-318), whose np.mean adds
+// the elements one after another from 0.0 and divides by the count
+__device__ __forceinline__ double seq_mean(const double *a, int n, int st) {
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) r += a[i * st];
+    return r / n;
 }
 
 // plane_basis_from_normal (utils.py:78-122)
@@ -201,7 +190,7 @@ __global__ __launch_bounds__(kTetBlock) void tet_pair_kernel(
             } else if (cnt >= 3) {
                 // order_points (:300-318): by the angle about the mean point
                 // (atan2_cr, correctly rounded), stable insertion sort
-                const double cx = np_sum(pts, cnt, 2) / cnt, cy = np_sum(pts + 1, cnt, 2) / cnt;
+                const double cx = seq_mean(pts, cnt, 2), cy = seq_mean(pts + 1, cnt, 2);
                 for (int q = 0; q < cnt; ++q) {
                     ang_s[q][lane] = atan2_cr(pts[2 * q + 1] - cy, pts[2 * q] - cx);
                     ord_s[q][lane] = (unsigned char)q;

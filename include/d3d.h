@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define D3D_ABI_VERSION 4
+#define D3D_ABI_VERSION 5
 
 /* error codes */
 #define D3D_OK 0
@@ -106,12 +106,14 @@ typedef struct d3d_scene {
  * iteration cap the reference loop lacks and the launch tuning.
  *
  * Work classes: d3d_gjk_distance_batch sorts pairs by the total number of
- * hull vertices n = n_A + n_B (0 for analytic colliders):
- *   class 0: n = 0, 1: n <= 32, 2: <= 128, 3: <= 512, 4: <= 1024,
- *   5: <= 2048, 6: <= 4096, 7: > 4096.
+ * hull vertices n = n_A + n_B (0 for analytic colliders) and by whether both
+ * colliders are hulls:
+ *   size class 0: n = 0, 1: n <= 32, 2: <= 128, 3: <= 512, 4: <= 1024,
+ *   5: <= 2048, 6: <= 4096, 7: > 4096;
+ *   work class = size class, plus 8 when both colliders are hulls (D3D_NUM_CLASSES).
  * lanes_per_pair[c] lanes (a power of two <= 64; 0 = built-in default
- * 1,1,1,1,2,4,8,16) work on each pair of class c (class 0: 1, 2 or 4; larger
- * values use 4).  For hull classes with 4 or more lanes, a pair whose hull
+ * 1,4,16,8,8,16,64,64 for one hull and 1,8,16,16,64,16,64,64 for two) work on
+ * each pair of class c (class 0: 1, 2 or 4; larger values use 4).  For hull classes with 4 or more lanes, a pair whose hull
  * vertices fit the group's LDS slot (44, 109, 240, 480 or 1008 vertices with
  * 4, 8, 16, 32 or 64 lanes) has them copied there once and is swept from LDS;
  * when both colliders are hulls the two halves of the group sweep A and B
@@ -121,6 +123,7 @@ typedef struct d3d_scene {
  * wavefront claims claim_factor * 64 / G pairs of a class at a time (1..8,
  * default 2; fewer when the class is small); epa_blocks_per_cu persistent EPA
  * wavefronts per compute unit in the fused pipeline (1..16, default 8). */
+#define D3D_NUM_CLASSES 16
 typedef struct d3d_gjk_params {
     double tolerance;            /* 1e-10 */
     double max_distance_squared; /* 1e5 */
@@ -129,7 +132,7 @@ typedef struct d3d_gjk_params {
     int32_t hull_serial_max;     /* intersection / support kernels: hulls with
                                     <= this many vertices are swept by their
                                     own lane, larger ones by the wavefront */
-    int32_t lanes_per_pair[8];
+    int32_t lanes_per_pair[D3D_NUM_CLASSES];
     int32_t blocks_per_cu;
     int32_t claim_factor;
     int32_t epa_blocks_per_cu;

@@ -1169,23 +1169,14 @@ void orc_atan2(const double *y, const double *x, int64_t n, double *out) {
     for (int64_t k = 0; k < n; ++k) out[k] = atan2_cr(y[k], x[k]);
 }
 
-/* numpy's pairwise sum of n doubles at stride st (numpy/core/src/umath/
- * loops_utils.h pairwise_sum for n <= 128: plain loop below 8, else eight
- * running sums combined as ((0+1)+(2+3))+((4+5)+(6+7)) then the tail). */
-static double np_sum(const double *a, int n, int st) {
-    if (n < 8) {
-        double r = 0.0;
-        for (int i = 0; i < n; ++i) r += a[i * st];
-        return r;
-    }
-    double r[8];
-    for (int j = 0; j < 8; ++j) r[j] = a[j * st];
-    int i;
-    for (i = 8; i < n - (n % 8); i += 8)
-        for (int j = 0; j < 8; ++j) r[j] += a[(i + j) * st];
-    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-    for (; i < n; ++i) res += a[i * st];
-    return res;
+/* np.mean of n doubles at stride st as order_points computes it: the function
+ * is @numba.njit-compiled (_tetrahedron_intersection.py:299-318), and numba's
+ * np.mean adds the elements one after another in iteration order, starting
+ * from 0.0, then divides by the count (numba/np/arraymath.py, array_mean). */
+static double seq_mean(const double *a, int n, int st) {
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) r += a[i * st];
+    return r / n;
 }
 
 /* plane_basis_from_normal (utils.py:78-122) */
@@ -1320,7 +1311,7 @@ static int tet_pair(const double *T1, const double *e1, const double *X1, const 
     if (np < 0) return TET_OVERFLOW | TET_CUTS;
     if (np < 3) return TET_CUTS;
     /* order_points (:300-318): by the angle (atan2_cr) around the mean point, stable */
-    const double cx = np_sum(&pts[0][0], np, 2) / np, cy = np_sum(&pts[0][1], np, 2) / np;
+    const double cx = seq_mean(&pts[0][0], np, 2), cy = seq_mean(&pts[0][1], np, 2);
     double ang[24];
     int ord[24];
     for (int k = 0; k < np; ++k) {

@@ -88,8 +88,9 @@ def test_tuned_tables_read_back(tmp_path):
                              "launch": {"blocks_per_cu": 2, "claim_factor": 4,
                                         "epa_blocks_per_cu": 8},
                              "serving_lanes_per_pair": [4, 8, 64, 32, 64, 32, 64, 64]}))
-    assert autotune.tuned_lanes(str(p)) == [1, 4, 8, 16, 32, 64, 64, 64]
-    assert autotune.tuned_lanes(str(p), kind="latency") == [4, 8, 64, 32, 64, 32, 64, 64]
+    # an 8-entry (per size class) table serves pairs of one and of two hulls
+    assert autotune.tuned_lanes(str(p)) == [1, 4, 8, 16, 32, 64, 64, 64] * 2
+    assert autotune.tuned_lanes(str(p), kind="latency") == [4, 8, 64, 32, 64, 32, 64, 64] * 2
     assert autotune.tuned_launch(str(p)) == {"blocks_per_cu": 2, "claim_factor": 4,
                                              "epa_blocks_per_cu": 8}
     assert autotune.tuned_launch(str(p), kind="latency") is None
@@ -98,3 +99,26 @@ def test_tuned_tables_read_back(tmp_path):
 def test_work_classes_match_the_device_rule():
     n = np.array([0, 1, 32, 33, 128, 129, 512, 513, 1024, 1025, 2048, 2049, 4096, 4097, 10 ** 6])
     assert list(autotune.work_class(n)) == [0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6, 7, 7]
+
+
+def test_pair_classes_split_pairs_of_two_hulls():
+    from distance3d_amd import _native
+    from distance3d_amd.scene import SceneBuilder
+    b = SceneBuilder()
+    rs = np.random.RandomState(0)
+    h40 = b.add_hull(rs.randn(40, 3))
+    h600 = b.add_hull(rs.randn(600, 3))
+    sph = b.add_sphere(np.zeros(3), 1.0)
+    scene = b.build()
+    pa = np.array([sph, h40, h40, h600, h40], dtype=np.int32)
+    pb = np.array([sph, sph, h40, sph, h600], dtype=np.int32)
+    # sizes 0, 40, 80, 600, 640 vertices -> size classes 0, 2, 2, 4, 4
+    assert list(autotune.pair_classes(scene, pa, pb)) == [0, 2, 10, 4, 12]
+    assert _native.expand_lanes(range(16)) == list(range(16))
+    assert _native.expand_lanes([1, 2, 3, 4, 5, 6, 7, 8]) == [1, 2, 3, 4, 5, 6, 7, 8] * 2
+    try:
+        _native.expand_lanes([1, 2, 3])
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("a 3-entry table was accepted")
