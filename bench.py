@@ -799,7 +799,8 @@ def main():
             "FP64 VALU operations per launch from rocprofv3 (SQ_INSTS_VALU_ADD/MUL/TRANS_F64 + "
             "2 x FMA_F64, x 64 lanes: issued lanes, an upper bound on useful FLOPs) over the "
             "kernel time measured in this run"))
-    for k in ("gjk_iterations", "effective_sweep", "epa_status_counts", "epa_unconverged"):
+    for k in ("stages_ms", "gjk_iterations", "effective_sweep", "epa_status_counts",
+              "epa_unconverged"):
         if k in head:
             result[k] = head[k]
     if "parity" in head:

@@ -45,7 +45,7 @@ struct EpaLds {
     // copies (-1: not cached)
     float cx[kEpaCache], cy[kEpaCache], cz[kEpaCache];
     int coff[2];
-    int split;  // both hulls cached: A and B swept by the two halves at once
+    int split;  // A and B evaluated by the two halves at once
     // the two collider records (scene layout), staged once per pair
     double recf[2][16];
     int32_t reci[2][4];
@@ -80,17 +80,22 @@ __device__ __forceinline__ SupportPair epa_support_pair(const d3d_scene &s, cons
     v3 a = mk(0.0, 0.0, 0.0), b = a;
     d = mk(wave_uniform(d.x), wave_uniform(d.y), wave_uniform(d.z));
     if (L.split) {
-        // both hulls cached: lanes 0..31 sweep A along d, lanes 32..63 sweep
-        // B along -d, at the same time; the supports are read from lanes 0
-        // and 32
+        // each collider analytic or a cached hull: lanes 0..31 evaluate A along
+        // d, lanes 32..63 B along -d, at the same time (a hull is swept from
+        // LDS by its half); the supports are read from lanes 0 and 32
         const int half = lane_id() >> 5;
         const Shape h = lds_shape(L, half);
         const v3 dir = vsel(half == 0, d, -d);
-        const v3 l = to_local(h, dir);
-        const int off = L.coff[half];
-        const int idx = lds_orig_argmax<32>(L.cx, L.cy, L.cz, off, h.nv, h.p[0], l, lane_id() & 31);
-        const v3 v = mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx]);
-        const v3 out = apply_margin(h, to_world(h, v), dir);
+        v3 out;
+        if (h.type == D3D_HULL) {
+            const v3 l = to_local(h, dir);
+            const int off = L.coff[half];
+            const int idx = lds_orig_argmax<32>(L.cx, L.cy, L.cz, off, h.nv, h.p[0], l, lane_id() & 31);
+            out = to_world(h, mk((double)L.cx[off + idx], (double)L.cy[off + idx], (double)L.cz[off + idx]));
+        } else {
+            out = support_analytic(h, dir);
+        }
+        out = apply_margin(h, out, dir);
         return SupportPair{readlane_v3(out, 0), readlane_v3(out, 32)};
     }
 #pragma unroll 1
@@ -393,7 +398,9 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             if (lane == 0) {
                 L.coff[0] = fit && na > 0 ? 0 : -1;
                 L.coff[1] = fit && nb > 0 ? na : -1;
-                L.split = fit && na > 0 && nb > 0;
+                // both halves can work alone: analytic, or a hull in the cache
+                L.split = (SA.type != D3D_HULL || (fit && na > 0)) &&
+                          (SB.type != D3D_HULL || (fit && nb > 0));
             }
         }
         // ---- seed: the GJK simplex rows (w, support on A, support on B) go
