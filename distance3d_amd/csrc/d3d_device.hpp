@@ -240,6 +240,26 @@ __device__ __forceinline__ Shape load_shape(const d3d_scene &s, int k) {
     return h;
 }
 
+// The FP64 block of record k (pose, parameters, margin) with a type already
+// known: for analytic records, whose supports read nothing else.
+__device__ __forceinline__ Shape load_shape_f64(const d3d_scene &s, int k, int type) {
+    Shape h;
+    const double *f = s.shape_f64 + 16 * (int64_t)k;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) h.R[j] = f[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        h.t[j] = f[9 + j];
+        h.p[j] = f[12 + j];
+    }
+    h.margin = f[15];
+    h.type = type;
+    h.nv = 0;
+    h.voff = 0;
+    h.voff64 = -1;
+    return h;
+}
+
 __device__ __forceinline__ Shape empty_shape() {
     Shape h;
 #pragma unroll
@@ -307,6 +327,49 @@ __device__ __forceinline__ v3 hull_vertex(const d3d_scene &s, int voff, int voff
     }
     const int64_t j = (int64_t)voff + idx;
     return mk((double)s.vx[j], (double)s.vy[j], (double)s.vz[j]);
+}
+
+// Copies the nv FP32 vertices of the hull at FP32-pool offset voff into the
+// LDS arrays cx, cy, cz at position at + (original index), by the N lanes of
+// a group (this lane: sub).  The pools are 16-byte aligned (checked by every
+// entry point), so the body of the range from the first multiple of four is
+// read with 16-byte loads, four vertices and their original indices per lane
+// per round; the up to three vertices before it and after it one at a time.
+// An original index outside [0, nv) (a malformed table) writes nothing.
+template <int N>
+__device__ __forceinline__ void fill_orig(const d3d_scene &s, int64_t voff, int nv, float *cx,
+                                          float *cy, float *cz, int at, int sub) {
+    auto put = [&](int o, float x, float y, float z) {
+        if ((unsigned)o < (unsigned)nv) {
+            cx[at + o] = x;
+            cy[at + o] = y;
+            cz[at + o] = z;
+        }
+    };
+    const int head = min(nv, (int)((4 - (voff & 3)) & 3));
+    const int nq = (nv - head) >> 2;
+    const int tail = head + 4 * nq;
+    // the head and tail: at most six single vertices
+    for (int k = sub; k < head + (nv - tail); k += N) {
+        const int j = k < head ? k : tail + (k - head);
+        const int64_t p = voff + j;
+        put(s.vorig ? s.vorig[p] : j, s.vx[p], s.vy[p], s.vz[p]);
+    }
+    const int64_t b = voff + head;
+    const float4 *__restrict__ qx = reinterpret_cast<const float4 *>(s.vx + b);
+    const float4 *__restrict__ qy = reinterpret_cast<const float4 *>(s.vy + b);
+    const float4 *__restrict__ qz = reinterpret_cast<const float4 *>(s.vz + b);
+    const int4 *__restrict__ qo = s.vorig ? reinterpret_cast<const int4 *>(s.vorig + b) : nullptr;
+#pragma unroll 2
+    for (int q = sub; q < nq; q += N) {
+        const float4 x = qx[q], y = qy[q], z = qz[q];
+        const int j = head + 4 * q;
+        const int4 o = qo ? qo[q] : make_int4(j, j + 1, j + 2, j + 3);
+        put(o.x, x.x, y.x, z.x);
+        put(o.y, x.y, y.y, z.y);
+        put(o.z, x.z, y.z, z.z);
+        put(o.w, x.w, y.w, z.w);
+    }
 }
 
 // A hull's vertex range and sweep metadata (see d3d.h for the clustered
@@ -737,6 +800,22 @@ __device__ __forceinline__ SupportPair support_pair(const d3d_scene &s, int ia, 
             out = support_analytic(h, dir);
         }
         out = apply_margin(h, out, dir);
+        a = vsel(k == 0, out, a);
+        b = vsel(k == 0, b, out);
+    }
+    return SupportPair{a, b};
+}
+
+// support_pair for two analytic records of known types ta, tb: only their
+// FP64 blocks are reloaded per call.
+__device__ __forceinline__ SupportPair support_pair_analytic(const d3d_scene &s, int ia, int ib,
+                                                             int ta, int tb, v3 d) {
+    v3 a = mk(0.0, 0.0, 0.0), b = a;
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const Shape h = load_shape_f64(s, opaque(k == 0 ? ia : ib), k == 0 ? ta : tb);
+        const v3 dir = vsel(k == 0, d, -d);
+        const v3 out = apply_margin(h, support_analytic(h, dir), dir);
         a = vsel(k == 0, out, a);
         b = vsel(k == 0, b, out);
     }

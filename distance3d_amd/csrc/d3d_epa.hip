@@ -378,22 +378,8 @@ __global__ __launch_bounds__(kEpaBlock) void epa_kernel(
             const int nb = SB.type == D3D_HULL && SB.voff64 < 0 ? SB.nv : 0;
             const bool fit = na + nb <= kEpaCache;
             if (fit) {
-                for (int j = lane; j < na; j += kEpaBlock) {
-                    const int o = s.vorig ? s.vorig[SA.voff + j] : j;
-                    if ((unsigned)o < (unsigned)na) {
-                        L.cx[o] = s.vx[SA.voff + j];
-                        L.cy[o] = s.vy[SA.voff + j];
-                        L.cz[o] = s.vz[SA.voff + j];
-                    }
-                }
-                for (int j = lane; j < nb; j += kEpaBlock) {
-                    const int o = s.vorig ? s.vorig[SB.voff + j] : j;
-                    if ((unsigned)o < (unsigned)nb) {
-                        L.cx[na + o] = s.vx[SB.voff + j];
-                        L.cy[na + o] = s.vy[SB.voff + j];
-                        L.cz[na + o] = s.vz[SB.voff + j];
-                    }
-                }
+                if (na > 0) fill_orig<kEpaBlock>(s, SA.voff, na, L.cx, L.cy, L.cz, 0, lane);
+                if (nb > 0) fill_orig<kEpaBlock>(s, SB.voff, nb, L.cx, L.cy, L.cz, na, lane);
             }
             if (lane == 0) {
                 L.coff[0] = fit && na > 0 ? 0 : -1;

@@ -443,18 +443,7 @@ template <int G>
 __device__ __forceinline__ void cache_fill(const d3d_scene &s, const Shape &h, float *cache,
                                            int at, int sub) {
     constexpr int C = cache_verts<G>();
-    const int64_t base = h.voff;
-#pragma unroll 2
-    for (int j = sub; j < h.nv; j += G) {
-        const int o = s.vorig ? s.vorig[base + j] : j;
-        const float x = s.vx[base + j], y = s.vy[base + j], z = s.vz[base + j];
-        // an original index outside the hull (a malformed table) writes nothing
-        if ((unsigned)o < (unsigned)h.nv) {
-            cache[at + o] = x;
-            cache[C + at + o] = y;
-            cache[2 * C + at + o] = z;
-        }
-    }
+    fill_orig<G>(s, h.voff, h.nv, cache, cache + C, cache + 2 * C, at, sub);
 }
 
 // Supports of records ia along d and ib along -d (see support_pair) for a
@@ -574,6 +563,8 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
     // record count, the offset of B's)
     bool bounded = false;
     int ncA = 0;
+    // analytic classes: the two record types, read once per pair
+    int ta = 0, tb = 0;
     bool have = false, exhausted = false;
     // The first chunk of every wavefront is fixed by its rank (wavefront index
     // minus `rot`, modulo the wavefront count): no atomic, so thousands of
@@ -631,6 +622,8 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                     gjk_init(g, pq);
                     if (ok) {
                         have = true;
+                        ta = A.type;
+                        tb = B.type;
                         if constexpr (kCache) {
                             const bool ha = A.type == D3D_HULL, hb = B.type == D3D_HULL;
                             nA = ha ? A.nv : 0;
@@ -685,6 +678,8 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                                                    reinterpret_cast<const float4 *>(cache), ncA);
                     else
                         w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
+                } else if constexpr (!kHull) {
+                    w = support_pair_analytic(s, ia, ib, ta, tb, g.d);
                 } else {
                     w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
                 }

@@ -61,6 +61,15 @@ inline int check_scene(const d3d_scene *s) {
         set_error("scene has clusters but a NULL cluster pointer");
         return D3D_E_INVALID;
     }
+    // the kernels read the int32 table, the FP32 pools, the original indices
+    // and the cluster records with 16-byte loads
+    const uintptr_t a16 = (uintptr_t)s->shape_i32 | (uintptr_t)s->vx | (uintptr_t)s->vy |
+                          (uintptr_t)s->vz | (uintptr_t)s->vorig | (uintptr_t)s->clusters;
+    const uintptr_t a8 = (uintptr_t)s->shape_f64 | (uintptr_t)s->v64;
+    if ((a16 & 15) != 0 || (a8 & 7) != 0) {
+        set_error("scene arrays must be 16-byte aligned (shape_f64 and v64: 8-byte)");
+        return D3D_E_INVALID;
+    }
     return D3D_OK;
 }
 

@@ -84,6 +84,9 @@ extern "C" {
  *                           vertex score (may be NULL when n_clusters == 0)
  *
  * d3d_cluster_scene builds a clustered copy of a scene on the host.
+ * Alignment: shape_i32, vx, vy, vz, vorig and clusters start on 16-byte
+ * boundaries, shape_f64 and v64 on 8-byte ones (device allocations are);
+ * entry points taking a scene return D3D_E_INVALID otherwise.
  */
 #define D3D_CLUSTER_SIZE 16
 

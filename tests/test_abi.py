@@ -72,10 +72,15 @@ def test_argument_errors_without_gpu():
     # before any pointer is used (the pointers below are never dereferenced)
     import ctypes
     sc = _native.d3d_scene()
-    sc.shape_f64 = sc.shape_i32 = 8
+    sc.shape_f64 = sc.shape_i32 = 16
     sc.n_shapes = 1
-    fake = ctypes.c_void_p(8)
+    fake = ctypes.c_void_p(16)
     args = [ctypes.byref(sc), fake, fake, 1, fake, fake]
+    # scene arrays read with 16-byte loads must be 16-byte aligned
+    sc.shape_i32 = 8
+    assert lib.d3d_epa_batch(*args, 64, 128, 1e-8, fake, fake, None, None, None, None) == invalid
+    assert b"aligned" in lib.d3d_last_error()
+    sc.shape_i32 = 16
     assert lib.d3d_epa_batch(*args, 0, 64, 1e-8, fake, fake, None, None, None, None) == invalid
     assert b"max_iterations" in lib.d3d_last_error()
     assert lib.d3d_epa_batch(*args, 64, 129, 1e-8, fake, fake, None, None, None, None) == invalid
