@@ -430,12 +430,66 @@ __host__ __device__ constexpr int cache_stride_words() {
     constexpr int room = (3072 - 48 * (kWave / G)) / (kWave / G);
     return G >= kWave ? room : ((room - G) / 64) * 64 + G;
 }
+// With 8 to 32 lanes per pair the last kRecWords words of a slot hold the
+// pair's two collider records (the FP64 blocks of A and B, then their int32
+// parts), staged once when the group takes the pair, so the supports read
+// them from LDS instead of reloading them from memory at every iteration
+// (a wavefront-wide group loads them into scalar registers instead).
+constexpr int kRecWords = 2 * 16 * 2 + 2 * 4;
+template <int G>
+__host__ __device__ constexpr bool slot_recs() {
+    return G >= 8 && G < kWave;
+}
+// words of a slot for vertices or cluster records
+template <int G>
+__host__ __device__ constexpr int cache_words() {
+    return cache_stride_words<G>() - (slot_recs<G>() ? kRecWords : 0);
+}
 template <int G>
 __host__ __device__ constexpr int cache_verts() {
-    return cache_stride_words<G>() / 3;
+    return cache_words<G>() / 3;
 }
 static_assert(cache_verts<4>() >= 32, "class 1 pairs fit a 4-lane slot");
 static_assert(cache_verts<64>() >= 1000, "one slot per wavefront holds 1000 vertices");
+static_assert(cache_words<8>() % 2 == 0 && cache_stride_words<8>() % 2 == 0 &&
+                  cache_words<16>() % 2 == 0 && cache_stride_words<16>() % 2 == 0,
+              "slot records are 8-byte aligned");
+
+// Collider record k (0: A, 1: B) of the pair in a slot (see slot_recs).
+__device__ __forceinline__ Shape slot_shape(const float *recs, int k) {
+    const double *f = reinterpret_cast<const double *>(recs) + 16 * k;
+    const int32_t *i = reinterpret_cast<const int32_t *>(recs + 64) + 4 * k;
+    Shape h;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) h.R[j] = f[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        h.t[j] = f[9 + j];
+        h.p[j] = f[12 + j];
+    }
+    h.margin = f[15];
+    h.type = i[0];
+    h.nv = i[1];
+    h.voff = i[2];
+    h.voff64 = i[3];
+    return h;
+}
+
+// The collider record k of a pair: from the slot when it holds them, else
+// from the scene by index (kUniform: the same record for the whole
+// wavefront, so a wavefront-wide group loads it into scalar registers).
+template <int G, bool kUniform>
+__device__ __forceinline__ Shape pair_shape(const d3d_scene &s, const float *recs, int k, int idx) {
+    if constexpr (slot_recs<G>()) {
+        (void)s;
+        (void)idx;
+        return slot_shape(recs, k);
+    } else {
+        (void)recs;
+        (void)k;
+        return load_shape(s, (kUniform && G == kWave) ? wave_uniform(idx) : opaque(idx));
+    }
+}
 
 // Copies the FP32 vertices of hull record h (validated) into the cache at
 // `at`, in original order (vorig), by the G lanes of a group.
@@ -456,10 +510,11 @@ __device__ __forceinline__ SupportPair support_pair_cached(const d3d_scene &s, i
                                                            bool two) {
     constexpr int C = cache_verts<G>();
     const float *cx = cache, *cy = cache + C, *cz = cache + 2 * C;
+    const float *recs = cache + cache_words<G>();
     if (two) {
         constexpr int H = G / 2;
         const bool upper = sub >= H;
-        const Shape h = load_shape(s, opaque(upper ? ib : ia));
+        const Shape h = pair_shape<G, false>(s, recs, upper ? 1 : 0, upper ? ib : ia);
         const v3 dir = vsel(upper, -d, d);
         const int off = upper ? nA : 0;
         const int j = lds_orig_argmax<H>(cx, cy, cz, off, h.nv, h.p[0], to_local(h, dir), sub & (H - 1));
@@ -474,7 +529,7 @@ __device__ __forceinline__ SupportPair support_pair_cached(const d3d_scene &s, i
     v3 a = mk(0.0, 0.0, 0.0), b = a;
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
-        const Shape h = load_shape(s, opaque(k == 0 ? ia : ib));
+        const Shape h = pair_shape<G, false>(s, recs, k, k == 0 ? ia : ib);
         const v3 dir = vsel(k == 0, d, -d);
         v3 out;
         if (h.type == D3D_HULL) {
@@ -510,10 +565,10 @@ __device__ __forceinline__ SupportPair support_pair_bounds(const d3d_scene &s, i
                                                            int sub, const float4 *slot, int ncA) {
     v3 a = mk(0.0, 0.0, 0.0), b = a;
     if (G == kWave) d = mk(wave_uniform(d.x), wave_uniform(d.y), wave_uniform(d.z));
+    const float *recs = reinterpret_cast<const float *>(slot) + cache_words<G>();
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
-        const int idx = k == 0 ? ia : ib;
-        const Shape h = load_shape(s, G == kWave ? wave_uniform(idx) : opaque(idx));
+        const Shape h = pair_shape<G, true>(s, recs, k, k == 0 ? ia : ib);
         const v3 dir = vsel(k == 0, d, -d);
         v3 out;
         if (h.type == D3D_HULL) {
@@ -634,7 +689,7 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                             ncA = ha ? (int)A.p[2] : 0;
                             const int ncB = hb ? (int)B.p[2] : 0;
                             bounded = !cached && ncA + ncB > 0 &&
-                                      4 * (ncA + ncB) <= cache_stride_words<G>();
+                                      4 * (ncA + ncB) <= cache_words<G>();
                             if (cached) {
                                 if (ha) cache_fill<G>(s, A, cache, 0, sub);
                                 if (hb) cache_fill<G>(s, B, cache, nA, sub);
@@ -642,6 +697,16 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                                 float4 *slot = reinterpret_cast<float4 *>(cache);
                                 if (ncA > 0) bounds_fill<G>(s, A, slot, 0, sub);
                                 if (ncB > 0) bounds_fill<G>(s, B, slot, ncA, sub);
+                            }
+                            if constexpr (slot_recs<G>()) {
+                                double *rf = reinterpret_cast<double *>(cache + cache_words<G>());
+                                int32_t *ri = reinterpret_cast<int32_t *>(rf + 32);
+                                for (int k = sub; k < 40; k += G) {
+                                    if (k < 32)
+                                        rf[k] = s.shape_f64[16 * (int64_t)(k < 16 ? ia : ib) + (k & 15)];
+                                    else
+                                        ri[k - 32] = s.shape_i32[4 * (int64_t)(k < 36 ? ia : ib) + (k & 3)];
+                                }
                             }
                             // the group's lanes read what the others wrote
                             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
