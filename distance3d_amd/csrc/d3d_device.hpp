@@ -415,6 +415,38 @@ __device__ __forceinline__ void top2(float sc, int j, float &m1, float &m2, int 
     }
 }
 
+// A lane's sweep state: its two best FP32 scores with their vertices and its
+// third best score (ties are kept: a score equal to m1 becomes m2, one equal
+// to m2 becomes m3).  When the certificate fails, the lanes whose m3 lies
+// below the rescan threshold hold all their candidates in i1, i2, so only
+// those are evaluated in FP64 instead of sweeping the hull again.
+struct Top3 {
+    float m1, m2, m3;
+    int i1, i2;
+    __device__ __forceinline__ void init() {
+        m1 = m2 = m3 = -INFINITY;
+        i1 = i2 = 0x7fffffff;
+    }
+    __device__ __forceinline__ void take(float sc, int j) {
+        const bool c1 = sc > m1, c2 = sc > m2;
+        m3 = c2 ? m2 : fmaxf(m3, sc);
+        i2 = c1 ? i1 : (c2 ? j : i2);
+        m2 = fmaxf(m2, fminf(sc, m1));
+        if (c1) {
+            m1 = sc;
+            i1 = j;
+        }
+    }
+    // true when the group's lanes hold every vertex scoring >= thr in i1, i2
+    // (a NaN or -inf threshold, or a NaN score, never qualifies)
+    template <int G>
+    __device__ __forceinline__ bool group_holds(float thr) const {
+        const bool over = !(m3 < thr) || (!(m1 < thr) && i1 == 0x7fffffff) ||
+                          (!(m2 < thr) && i2 == 0x7fffffff);
+        return group_bits<G>(__ballot(over), lane_id()) == 0ull;
+    }
+};
+
 // Argmax over one hull, vertices[argmax(vertices . l)] with the first
 // (original) index on ties (colliders.py:131-132, np.argmax), evaluated by
 // the G lanes of a group (G a power of two <= 64; all lanes of the group
@@ -551,8 +583,8 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
     const int nv = H.nv;
     const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
     const double E = 8.0 * 5.9604644775390625e-08 * H.L * linf;
-    float m1 = -INFINITY, m2 = -INFINITY;
-    int i1 = 0x7fffffff;
+    Top3 T;
+    T.init();
     auto score = [&](int j) {
         return __builtin_fmaf(pz[j], dz, __builtin_fmaf(py[j], dy, px[j] * dx));
     };
@@ -561,12 +593,12 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
         int k = sub;
         for (; k + 3 * G < nv; k += 4 * G) {
             const float s0 = score(k), s1 = score(k + G), s2 = score(k + 2 * G), s3 = score(k + 3 * G);
-            top2(s0, k, m1, m2, i1);
-            top2(s1, k + G, m1, m2, i1);
-            top2(s2, k + 2 * G, m1, m2, i1);
-            top2(s3, k + 3 * G, m1, m2, i1);
+            T.take(s0, k);
+            T.take(s1, k + G);
+            T.take(s2, k + 2 * G);
+            T.take(s3, k + 3 * G);
         }
-        for (; k < nv; k += G) top2(score(k), k, m1, m2, i1);
+        for (; k < nv; k += G) T.take(score(k), k);
     } else {
         const float4 *__restrict__ cl =
             kBounds ? bounds : reinterpret_cast<const float4 *>(s.clusters) + H.coff;
@@ -595,9 +627,9 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
             const int cnt = min(K, nv - j0);
 #pragma unroll
             for (int o = sub % K; o < K; o += (G < K ? G : K))
-                if ((G <= K || sub < K) && o < cnt) top2(score(j0 + o), j0 + o, m1, m2, i1);
+                if ((G <= K || sub < K) && o < cnt) T.take(score(j0 + o), j0 + o);
         }
-        float thr = __double2float_rd((double)group_maxf<G>(m1) - 2.5 * E);
+        float thr = __double2float_rd((double)group_maxf<G>(T.m1) - 2.5 * E);
         // every other cluster whose bound reaches the threshold is scanned whole
         // by the lane that owns it (16-byte aligned, four vertices per load);
         // the threshold only rises, with the lane's own running maximum
@@ -612,21 +644,32 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
                 for (int c = 0; c < K / 4; ++c) {
                     const float4 x = qx[c], y = qy[c], z = qz[c];
                     const int j = j0 + 4 * c;
-                    top2(__builtin_fmaf(z.x, dz, __builtin_fmaf(y.x, dy, x.x * dx)), j, m1, m2, i1);
-                    top2(__builtin_fmaf(z.y, dz, __builtin_fmaf(y.y, dy, x.y * dx)), j + 1, m1, m2, i1);
-                    top2(__builtin_fmaf(z.z, dz, __builtin_fmaf(y.z, dy, x.z * dx)), j + 2, m1, m2, i1);
-                    top2(__builtin_fmaf(z.w, dz, __builtin_fmaf(y.w, dy, x.w * dx)), j + 3, m1, m2, i1);
+                    T.take(__builtin_fmaf(z.x, dz, __builtin_fmaf(y.x, dy, x.x * dx)), j);
+                    T.take(__builtin_fmaf(z.y, dz, __builtin_fmaf(y.y, dy, x.y * dx)), j + 1);
+                    T.take(__builtin_fmaf(z.z, dz, __builtin_fmaf(y.z, dy, x.z * dx)), j + 2);
+                    T.take(__builtin_fmaf(z.w, dz, __builtin_fmaf(y.w, dy, x.w * dx)), j + 3);
                 }
             } else {
-                for (int j = j0; j < nv; ++j) top2(score(j), j, m1, m2, i1);
+                for (int j = j0; j < nv; ++j) T.take(score(j), j);
             }
-            thr = fmaxf(thr, __double2float_rd((double)m1 - 2.5 * E));
+            thr = fmaxf(thr, __double2float_rd((double)T.m1 - 2.5 * E));
         }
     }
+    float m1 = T.m1, m2 = T.m2;
+    int i1 = T.i1;
     group_top2<G>(m1, m2, i1);
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
-
-    return hull_rescan<G, kBounds>(s, H, l, sub, __double2float_rd((double)m1 - 2.5 * E), bounds);
+    float thr = __double2float_rd((double)m1 - 2.5 * E);
+    if (thr != thr) thr = -INFINITY;
+    if (T.group_holds<G>(thr)) {
+        // every candidate is some lane's i1 or i2: FP64 scores of those only
+        ArgmaxF64 A;
+        A.init();
+        if (!(T.m1 < thr)) rescan_take(s, H, l, T.i1, A);
+        if (!(T.m2 < thr)) rescan_take(s, H, l, T.i2, A);
+        return A.group_result<G>();
+    }
+    return hull_rescan<G, kBounds>(s, H, l, sub, thr, bounds);
 }
 
 // Argmax over a hull whose FP32 vertices sit in LDS in their ORIGINAL order
@@ -642,17 +685,30 @@ __device__ __forceinline__ int lds_orig_argmax(const float *cx, const float *cy,
     const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
     const double linf = fmax(fabs(l.x), fmax(fabs(l.y), fabs(l.z)));
     const double E = 8.0 * 5.9604644775390625e-08 * Lmax * linf;
-    float m1 = -INFINITY, m2 = -INFINITY;
-    int i1 = 0x7fffffff;
+    Top3 T;
+    T.init();
     for (int j = sub; j < nv; j += G)
-        top2(__builtin_fmaf(cz[off + j], dz, __builtin_fmaf(cy[off + j], dy, cx[off + j] * dx)), j, m1,
-             m2, i1);
+        T.take(__builtin_fmaf(cz[off + j], dz, __builtin_fmaf(cy[off + j], dy, cx[off + j] * dx)), j);
+    float m1 = T.m1, m2 = T.m2;
+    int i1 = T.i1;
     group_top2<G>(m1, m2, i1);
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
     float thr = __double2float_rd((double)m1 - 2.5 * E);
     if (thr != thr) thr = -INFINITY;
     ArgmaxF64 A;
     A.init();
+    if (T.group_holds<G>(thr)) {
+        // every candidate is some lane's i1 or i2: FP64 scores of those only
+        if (!(T.m1 < thr))
+            A.take((double)cx[off + T.i1] * l.x + (double)cy[off + T.i1] * l.y +
+                       (double)cz[off + T.i1] * l.z,
+                   T.i1, T.i1);
+        if (!(T.m2 < thr))
+            A.take((double)cx[off + T.i2] * l.x + (double)cy[off + T.i2] * l.y +
+                       (double)cz[off + T.i2] * l.z,
+                   T.i2, T.i2);
+        return A.group_result<G>();
+    }
     for (int j = sub; j < nv; j += G) {
         const float x = cx[off + j], y = cy[off + j], z = cz[off + j];
         if (!(__builtin_fmaf(z, dz, __builtin_fmaf(y, dy, x * dx)) < thr))
