@@ -447,6 +447,16 @@ struct Top3 {
     }
 };
 
+// One score into a sweep state: with kKeep as Top3, else as top2 (m3 and i2
+// are not kept).
+template <bool kKeep>
+__device__ __forceinline__ void sweep_take(Top3 &T, float sc, int j) {
+    if constexpr (kKeep)
+        T.take(sc, j);
+    else
+        top2(sc, j, T.m1, T.m2, T.i1);
+}
+
 // Argmax over one hull, vertices[argmax(vertices . l)] with the first
 // (original) index on ties (colliders.py:131-132, np.argmax), evaluated by
 // the G lanes of a group (G a power of two <= 64; all lanes of the group
@@ -571,8 +581,12 @@ __device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H,
 }
 
 // With kBounds, `bounds` is a copy of the hull's cluster records (in LDS)
-// read instead of the scene's; the result is the same.
-template <int G, bool kBounds = false>
+// read instead of the scene's; the result is the same.  kKeep: the lanes keep
+// their candidates (Top3) so a failed certificate usually evaluates only
+// those; without it (one-lane sweeps of whole hulls, where the extra work per
+// vertex costs more than the rescans save) a failed certificate always
+// sweeps again.
+template <int G, bool kBounds = false, bool kKeep = true>
 __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H, v3 l, int sub,
                                            const float4 *bounds = nullptr) {
     constexpr int K = D3D_CLUSTER_SIZE;
@@ -593,12 +607,12 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
         int k = sub;
         for (; k + 3 * G < nv; k += 4 * G) {
             const float s0 = score(k), s1 = score(k + G), s2 = score(k + 2 * G), s3 = score(k + 3 * G);
-            T.take(s0, k);
-            T.take(s1, k + G);
-            T.take(s2, k + 2 * G);
-            T.take(s3, k + 3 * G);
+            sweep_take<kKeep>(T, s0, k);
+            sweep_take<kKeep>(T, s1, k + G);
+            sweep_take<kKeep>(T, s2, k + 2 * G);
+            sweep_take<kKeep>(T, s3, k + 3 * G);
         }
-        for (; k < nv; k += G) T.take(score(k), k);
+        for (; k < nv; k += G) sweep_take<kKeep>(T, score(k), k);
     } else {
         const float4 *__restrict__ cl =
             kBounds ? bounds : reinterpret_cast<const float4 *>(s.clusters) + H.coff;
@@ -627,7 +641,7 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
             const int cnt = min(K, nv - j0);
 #pragma unroll
             for (int o = sub % K; o < K; o += (G < K ? G : K))
-                if ((G <= K || sub < K) && o < cnt) T.take(score(j0 + o), j0 + o);
+                if ((G <= K || sub < K) && o < cnt) sweep_take<kKeep>(T, score(j0 + o), j0 + o);
         }
         float thr = __double2float_rd((double)group_maxf<G>(T.m1) - 2.5 * E);
         // every other cluster whose bound reaches the threshold is scanned whole
@@ -644,13 +658,13 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
                 for (int c = 0; c < K / 4; ++c) {
                     const float4 x = qx[c], y = qy[c], z = qz[c];
                     const int j = j0 + 4 * c;
-                    T.take(__builtin_fmaf(z.x, dz, __builtin_fmaf(y.x, dy, x.x * dx)), j);
-                    T.take(__builtin_fmaf(z.y, dz, __builtin_fmaf(y.y, dy, x.y * dx)), j + 1);
-                    T.take(__builtin_fmaf(z.z, dz, __builtin_fmaf(y.z, dy, x.z * dx)), j + 2);
-                    T.take(__builtin_fmaf(z.w, dz, __builtin_fmaf(y.w, dy, x.w * dx)), j + 3);
+                    sweep_take<kKeep>(T, __builtin_fmaf(z.x, dz, __builtin_fmaf(y.x, dy, x.x * dx)), j);
+                    sweep_take<kKeep>(T, __builtin_fmaf(z.y, dz, __builtin_fmaf(y.y, dy, x.y * dx)), j + 1);
+                    sweep_take<kKeep>(T, __builtin_fmaf(z.z, dz, __builtin_fmaf(y.z, dy, x.z * dx)), j + 2);
+                    sweep_take<kKeep>(T, __builtin_fmaf(z.w, dz, __builtin_fmaf(y.w, dy, x.w * dx)), j + 3);
                 }
             } else {
-                for (int j = j0; j < nv; ++j) T.take(score(j), j);
+                for (int j = j0; j < nv; ++j) sweep_take<kKeep>(T, score(j), j);
             }
             thr = fmaxf(thr, __double2float_rd((double)T.m1 - 2.5 * E));
         }
@@ -661,7 +675,7 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
     if ((double)m1 - (double)m2 > 2.0 * E) return i1;
     float thr = __double2float_rd((double)m1 - 2.5 * E);
     if (thr != thr) thr = -INFINITY;
-    if (T.group_holds<G>(thr)) {
+    if (kKeep && T.group_holds<G>(thr)) {
         // every candidate is some lane's i1 or i2: FP64 scores of those only
         ArgmaxF64 A;
         A.init();
@@ -891,7 +905,7 @@ __device__ __forceinline__ v3 support_wave(const d3d_scene &s, const Shape &h, v
     if (want && h.type == D3D_HULL) l = to_local(h, d);
     if (want && !coop) {
         if (h.type == D3D_HULL) {
-            idx = hull_argmax<1>(s, hull_ref(h), l, 0);
+            idx = hull_argmax<1, false, false>(s, hull_ref(h), l, 0);
         } else {
             out = support_analytic(h, d);
         }
@@ -902,7 +916,7 @@ __device__ __forceinline__ v3 support_wave(const d3d_scene &s, const Shape &h, v
         const int j = __ffsll((long long)m) - 1;
         m &= m - 1;
         const v3 lj = mk(readlane_f64(l.x, j), readlane_f64(l.y, j), readlane_f64(l.z, j));
-        const int best = hull_argmax<kWave>(s, readlane_hull(H, j), lj, lane_id());
+        const int best = hull_argmax<kWave, false, false>(s, readlane_hull(H, j), lj, lane_id());
         if (lane_id() == j) idx = best;
     }
     if (want && h.type == D3D_HULL) out = to_world(h, hull_vertex(s, h.voff, h.voff64, idx));
