@@ -1104,21 +1104,17 @@ __device__ __forceinline__ int outside_planes_mask(v3 a, v3 b, v3 c, v3 d) {
 }
 
 // closest_point_tetrahedron (_gjk_jolt.py:573-631) and, for three points,
-// closest_point_triangle: the faces ABC, ACD, ADB, BDC are visited in order by
-// one non-unrolled loop (a single copy of the triangle code keeps register
-// pressure bounded); face 0 always sets the result, later faces only when
-// strictly closer; the triangle's feature bits are remapped to the
+// closest_point_triangle: the faces ABC, ACD, ADB, BDC are visited in order,
+// each by its own inlined copy of the triangle code with its vertices named
+// directly (a loop over the faces picked them with a dozen selects per
+// coordinate and was slower); face 0 always sets the result, later faces only
+// when strictly closer; the triangle's feature bits are remapped to the
 // tetrahedron's vertex order.
 __device__ __forceinline__ v3 cp_faces(v3 y0, v3 y1, v3 y2, v3 y3, int mask, int &set) {
     int s = 15;
     v3 cp = mk(0.0, 0.0, 0.0);
     double best = kMaxFloat;
-#pragma unroll 1
-    for (int f = 0; f < 4; ++f) {
-        if (!((mask >> f) & 1)) continue;
-        const v3 a = vsel(f == 3, y1, y0);
-        const v3 b = sel4(f, y1, y2, y3, y3);
-        const v3 c = sel4(f, y2, y3, y1, y2);
+    auto face = [&](int f, v3 a, v3 b, v3 c) {
         int ns;
         const v3 q = cp_triangle(a, b, c, ns);
         const double dd = dot(q, q);
@@ -1130,7 +1126,11 @@ __device__ __forceinline__ v3 cp_faces(v3 y0, v3 y1, v3 y2, v3 y3, int mask, int
               : (f == 2) ? (ns & 1) + ((ns & 2) << 2) + ((ns & 4) >> 1)
                          : ((ns & 1) << 1) + ((ns & 2) << 2) + (ns & 4);
         }
-    }
+    };
+    if (mask & 1) face(0, y0, y1, y2);
+    if (mask & 2) face(1, y0, y2, y3);
+    if (mask & 4) face(2, y0, y3, y1);
+    if (mask & 8) face(3, y1, y3, y2);
     set = s;
     return cp;
 }
