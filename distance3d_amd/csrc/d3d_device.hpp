@@ -1021,7 +1021,8 @@ __device__ __forceinline__ v3 cp_line(v3 a, v3 b, int &set) {
 // closest_point_triangle (_gjk_jolt.py:415-523)
 __device__ __forceinline__ v3 cp_triangle(v3 a, v3 b, v3 c, int &set) {
     const v3 ab = b - a, ac = c - a, bc = c - b;
-    const v3 n = vsel(dot(bc, bc) < dot(ac, ac), cross(ab, bc), cross(ab, ac));
+    // the normal from the shorter of bc, ac (one cross product of the chosen edge)
+    const v3 n = cross(ab, vsel(dot(bc, bc) < dot(ac, ac), bc, ac));
     const double nn = dot(n, n);
     if (nn < kEpsSqr) {
         int s, ns;
@@ -1169,10 +1170,10 @@ struct CompactMap {
     bool moved;
 };
 
-// update_simplex_y / update_simplex_ypq on Y (_gjk_jolt.py:643-664); slots >=
-// the new count are left as they were.  Returns the map to apply to P, Q.
-__device__ __forceinline__ CompactMap simplex_compact(Simplex &S, int set) {
-    set &= (1 << S.n) - 1;
+// The compaction map of update_simplex_ypq (_gjk_jolt.py:654-664) for the
+// rows of `set` among the first n.
+__device__ __forceinline__ CompactMap compact_map(int n, int set) {
+    set &= (1 << n) - 1;
     CompactMap c;
     c.m = __popc(set);
     c.k0 = 0;
@@ -1185,6 +1186,15 @@ __device__ __forceinline__ CompactMap simplex_compact(Simplex &S, int set) {
         c.k1 = r1 ? __ffs(r1) - 1 : 3;
         const int r2 = r1 & (r1 - 1);
         c.k2 = r2 ? __ffs(r2) - 1 : 3;
+    }
+    return c;
+}
+
+// update_simplex_y / update_simplex_ypq on Y (_gjk_jolt.py:643-664); slots >=
+// the new count are left as they were.  Returns the map to apply to P, Q.
+__device__ __forceinline__ CompactMap simplex_compact(Simplex &S, int set) {
+    const CompactMap c = compact_map(S.n, set);
+    if (c.moved) {
         const v3 y0 = sel4(c.k0, S.Y0, S.Y1, S.Y2, S.Y3);
         const v3 y1 = sel4(c.k1, S.Y0, S.Y1, S.Y2, S.Y3);
         const v3 y2 = sel4(c.k2, S.Y0, S.Y1, S.Y2, S.Y3);
@@ -1194,6 +1204,29 @@ __device__ __forceinline__ CompactMap simplex_compact(Simplex &S, int set) {
     }
     S.n = c.m;
     return c;
+}
+
+// max_y_length_squared (_gjk_jolt.py:634-640) of the simplex that compacting
+// S to the rows of `set` leaves, read from S before the compaction: the kept
+// rows in order (compaction keeps their order), row 0 alone when none is kept
+// (it stays in place then).
+__device__ __forceinline__ double max_y_len_sq_kept(const Simplex &S, int set) {
+    set &= (1 << S.n) - 1;
+    if (set == 0) return dot(S.Y0, S.Y0);
+    double m = 0.0;
+    bool any = false;
+    auto acc = [&](bool keep, v3 y) {
+        if (keep) {
+            const double l = dot(y, y);
+            m = !any ? l : ((l > m) ? l : m);
+            any = true;
+        }
+    };
+    acc(set & 1, S.Y0);
+    acc(set & 2, S.Y1);
+    acc(set & 4, S.Y2);
+    acc(set & 8, S.Y3);
+    return m;
 }
 
 // max_y_length_squared (_gjk_jolt.py:634-640)

@@ -280,8 +280,13 @@ __device__ __forceinline__ bool gjk_distance_step(GjkState &g, const PQ &pq, v3 
         g.state = D3D_INTERSECT;
         return false;
     }
-    pq.compact(simplex_compact(g.S, set));
-    if (g.vv <= tol2 || g.vv <= kEps * max_y_len_sq(g.S)) {
+    // compaction moves the P/Q rows only: the next step rebuilds Y from them,
+    // and the test below reads the kept rows of Y where they are
+    const double ymax = max_y_len_sq_kept(g.S, set);
+    const CompactMap cm = compact_map(g.S.n, set);
+    pq.compact(cm);
+    g.S.n = cm.m;
+    if (g.vv <= tol2 || g.vv <= kEps * ymax) {
         g.vv = 0.0;
         g.state = D3D_INTERSECT;
         return false;
