@@ -877,11 +877,11 @@ __device__ __forceinline__ SupportPair support_pair(const d3d_scene &s, int ia, 
 }
 
 // support_pair for two analytic records of known types ta, tb: only their
-// FP64 blocks are reloaded per call.
+// FP64 blocks are reloaded per call, both records' loads issued together.
 __device__ __forceinline__ SupportPair support_pair_analytic(const d3d_scene &s, int ia, int ib,
                                                              int ta, int tb, v3 d) {
     v3 a = mk(0.0, 0.0, 0.0), b = a;
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < 2; ++k) {
         const Shape h = load_shape_f64(s, opaque(k == 0 ? ia : ib), k == 0 ? ta : tb);
         const v3 dir = vsel(k == 0, d, -d);
