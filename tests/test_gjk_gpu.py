@@ -147,6 +147,45 @@ def test_hull_argmax_ties_take_first_index(gpu):
     assert np.array_equal(out, oracle.support(host, ids, dirs))
 
 
+def test_tied_hull_vertices_every_lane_count(gpu):
+    """Hulls whose support vertices tie exactly (duplicated corners, in one
+    lane's stride or in several) drive the GJK sweeps' failed certificates
+    through both rescans: the kept candidates of each lane when they cover the
+    threshold, the full sweep when a lane holds three or more tied scores.
+    Every lane count and layout must give the oracle's bits (np.argmax's first
+    index on ties)."""
+    from distance3d_amd.scene import DeviceScene
+    rs = np.random.RandomState(11)
+    cube = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], float)
+    inner = rs.uniform(-0.5, 0.5, (100, 3))
+    hulls = [np.vstack([cube] * 40),                     # every corner 40 times
+             np.vstack([cube, inner, cube[::-1]]),        # every corner twice, far apart
+             np.vstack([inner[:20], cube, cube]),          # twice, eight apart
+             np.vstack([cube * 0.5, inner * 0.2, cube * 0.5 + 1e-9])]  # near ties
+    b = SceneBuilder()
+    hid = [b.add_hull(h) for h in hulls]
+    partners = []
+    for k in range(24):
+        c = rs.uniform(-3.0, 3.0, 3)
+        pose = np.eye(4)
+        pose[:3, 3] = c
+        partners.append(b.add_sphere(c, 0.3 + 0.5 * rs.rand()))
+        partners.append(b.add_box(pose, rs.uniform(0.2, 1.5, 3)))
+        partners.append(b.add_capsule(pose, 0.2, 1.0))
+    host = b.build()
+    pa = np.array([h for h in hid for _ in partners] + [hid[0], hid[1], hid[2]], dtype=np.int32)
+    pb = np.array(partners * len(hid) + [hid[1], hid[2], hid[3]], dtype=np.int32)
+    o = oracle.gjk_distance(host, pa, pb)
+    for cmin in (0, 64):
+        dev = DeviceScene(host, cluster_min=cmin)
+        for g in (1, 2, 4, 8, 16, 32, 64):
+            r = kernels.gjk_distance_batch(dev, pa, pb, lanes_per_pair=[1] + [g] * 15)
+            assert np.array_equal(_np(r.status), o["status"]), (cmin, g)
+            assert np.array_equal(_np(r.dist), o["dist"]), (cmin, g)
+            assert np.array_equal(_np(r.cp_a), o["cp_a"], equal_nan=True), (cmin, g)
+            assert np.array_equal(_np(r.cp_b), o["cp_b"], equal_nan=True), (cmin, g)
+
+
 def test_aabbs_bitwise(gpu):
     w = workloads.mixed_pairs(seed=7, n_pairs=3000)
     ids = np.arange(w.scene.n_shapes, dtype=np.int32)
