@@ -87,6 +87,35 @@ def test_epa_mixed_primitives_parity(gpu):
     _check(r, o)
 
 
+def test_epa_tied_hull_vertices_bitwise(gpu):
+    """Hulls whose support vertices tie exactly (duplicated corners) make EPA's
+    cached sweeps fail their certificate along many face normals; both
+    rescans (the lanes' kept candidates, or the full sweep) must reproduce
+    the oracle's first-index choices bit for bit."""
+    rs = np.random.RandomState(13)
+    cube = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], float)
+    inner = rs.uniform(-0.5, 0.5, (100, 3))
+    b = SceneBuilder()
+    hid = [b.add_hull(np.vstack([cube] * 40)),
+           b.add_hull(np.vstack([cube, inner, cube[::-1]])),
+           b.add_hull(np.vstack([inner[:20], cube, cube]) * 0.8)]
+    partners = []
+    for k in range(40):
+        T = gb.random_pose(rs, 0.8)
+        partners.append(b.add_box(T, rs.uniform(0.5, 1.5, 3)))
+        partners.append(b.add_sphere(T[:3, 3], rs.uniform(0.3, 0.9)))
+    host = b.build()
+    pa = np.array([h for h in hid for _ in partners] + [hid[0], hid[0], hid[1]], dtype=np.int32)
+    pb = np.array(partners * len(hid) + [hid[1], hid[2], hid[2]], dtype=np.int32)
+    r, o, idx = _epa_both(host, pa, pb)
+    assert len(idx) > 100
+    assert np.array_equal(_np(r.status), o["status"])
+    assert np.array_equal(_np(r.depth), o["depth"])
+    assert np.array_equal(_np(r.normal), o["normal"])
+    assert np.array_equal(_np(r.cp_a), o["cp_a"])
+    assert np.array_equal(_np(r.iterations), o["iterations"])
+
+
 def test_epa_small_hulls_vs_minkowski_brute_force(gpu):
     rs = np.random.RandomState(26)
     b = SceneBuilder()
