@@ -396,6 +396,31 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long m, i
     return (m >> (lane & ~(G - 1))) & ((1ull << G) - 1ull);
 }
 
+// Exclusive prefix sum of x over the G lanes of a group (lane order); total
+// receives the group's sum in every lane.
+template <int G>
+__device__ __forceinline__ int group_excl_scan(int x, int &total) {
+    int pre = 0, tot = x;
+    for_butterfly<G>([&](auto o) {
+        constexpr int OFF = decltype(o)::value;
+        int a, b;
+        xpair<OFF>(tot, a, b);
+        const bool upper = (lane_id() & OFF) != 0;
+        int lo, hi;
+        if constexpr (OFF <= 8) {
+            lo = upper ? b : a;
+            hi = upper ? a : b;
+        } else {
+            lo = a;
+            hi = b;
+        }
+        if (upper) pre += lo;
+        tot = lo + hi;
+    });
+    total = tot;
+    return pre;
+}
+
 template <int G>
 __device__ __forceinline__ float group_maxf(float x) {
     for_butterfly<G>([&](auto o) {
@@ -588,7 +613,8 @@ __device__ __forceinline__ int hull_rescan(const d3d_scene &s, const HullRef &H,
 // sweeps again.
 template <int G, bool kBounds = false, bool kKeep = true>
 __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H, v3 l, int sub,
-                                           const float4 *bounds = nullptr) {
+                                           const float4 *bounds = nullptr, int *list = nullptr,
+                                           int list_cap = 0) {
     constexpr int K = D3D_CLUSTER_SIZE;
     const float dx = (float)l.x, dy = (float)l.y, dz = (float)l.z;
     const float *__restrict__ px = s.vx + H.voff;
@@ -644,10 +670,46 @@ __device__ __forceinline__ int hull_argmax(const d3d_scene &s, const HullRef &H,
                 if ((G <= K || sub < K) && o < cnt) sweep_take<kKeep>(T, score(j0 + o), j0 + o);
         }
         float thr = __double2float_rd((double)group_maxf<G>(T.m1) - 2.5 * E);
-        // every other cluster whose bound reaches the threshold is scanned whole
-        // by the lane that owns it (16-byte aligned, four vertices per load);
-        // the threshold only rises, with the lane's own running maximum
-        for (int k = sub; k < nc; k += G) {
+        bool coop = false;
+        if constexpr (kBounds && G >= 8) coop = list != nullptr && nc <= list_cap && nc <= 32 * G;
+        if (coop) {
+            // every other cluster whose bound reaches the threshold goes into the
+            // group's list (lane order), then the group's lanes take its
+            // clusters' vertex quads, so all their loads go out at once
+            unsigned mask = 0u;
+            int cnt = 0;
+            for (int i = 0, k = sub; k < nc; ++i, k += G) {
+                if (k != ks && !(bound(k) < thr)) {
+                    mask |= 1u << i;
+                    ++cnt;
+                }
+            }
+            int total;
+            int pos = group_excl_scan<G>(cnt, total);
+            for (unsigned m = mask; m != 0u; m &= m - 1u) list[pos++] = sub + (__ffs((int)m) - 1) * G;
+            // the group's lanes read what the others wrote
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 2
+            for (int t = sub; t < 4 * total; t += G) {
+                const int j = list[t >> 2] * K + 4 * (t & 3);
+                if (j + 4 <= nv) {
+                    const float4 x = *reinterpret_cast<const float4 *>(px + j);
+                    const float4 y = *reinterpret_cast<const float4 *>(py + j);
+                    const float4 z = *reinterpret_cast<const float4 *>(pz + j);
+                    sweep_take<kKeep>(T, __builtin_fmaf(z.x, dz, __builtin_fmaf(y.x, dy, x.x * dx)), j);
+                    sweep_take<kKeep>(T, __builtin_fmaf(z.y, dz, __builtin_fmaf(y.y, dy, x.y * dx)), j + 1);
+                    sweep_take<kKeep>(T, __builtin_fmaf(z.z, dz, __builtin_fmaf(y.z, dy, x.z * dx)), j + 2);
+                    sweep_take<kKeep>(T, __builtin_fmaf(z.w, dz, __builtin_fmaf(y.w, dy, x.w * dx)), j + 3);
+                } else {
+                    for (int e = j; e < nv; ++e) sweep_take<kKeep>(T, score(e), e);
+                }
+            }
+        }
+        // otherwise every other cluster whose bound reaches the threshold is
+        // scanned whole by the lane that owns it (16-byte aligned, four vertices
+        // per load); the threshold only rises, with the lane's own running maximum
+        for (int k = sub; !coop && k < nc; k += G) {
             if (k == ks || bound(k) < thr) continue;
             const int j0 = k * K;
             if (j0 + K <= nv) {

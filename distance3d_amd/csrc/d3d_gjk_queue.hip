@@ -567,7 +567,8 @@ __device__ __forceinline__ void bounds_fill(const d3d_scene &s, const Shape &h, 
 // from the pools.  Same results as support_pair.
 template <int G>
 __device__ __forceinline__ SupportPair support_pair_bounds(const d3d_scene &s, int ia, int ib, v3 d,
-                                                           int sub, const float4 *slot, int ncA) {
+                                                           int sub, const float4 *slot, int ncA,
+                                                           int nrec) {
     v3 a = mk(0.0, 0.0, 0.0), b = a;
     if (G == kWave) d = mk(wave_uniform(d.x), wave_uniform(d.y), wave_uniform(d.z));
     const float *recs = reinterpret_cast<const float *>(slot) + cache_words<G>();
@@ -579,7 +580,10 @@ __device__ __forceinline__ SupportPair support_pair_bounds(const d3d_scene &s, i
         if (h.type == D3D_HULL) {
             const v3 l = to_local(h, dir);
             const HullRef H = hull_ref(h);
-            const int hi = H.nc > 0 ? hull_argmax<G, true>(s, H, l, sub, slot + (k == 0 ? 0 : ncA))
+            // the slot's words past the cluster records hold the survivor list
+            int *list = const_cast<int *>(reinterpret_cast<const int *>(slot + nrec));
+            const int hi = H.nc > 0 ? hull_argmax<G, true>(s, H, l, sub, slot + (k == 0 ? 0 : ncA),
+                                                           list, cache_words<G>() - 4 * nrec)
                                     : hull_argmax<G>(s, H, l, sub);
             out = to_world(h, hull_vertex(s, h.voff, h.voff64, hi));
         } else {
@@ -622,7 +626,7 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
     // otherwise, pairs with their hulls' cluster records in the slot (ncA: A's
     // record count, the offset of B's)
     bool bounded = false;
-    int ncA = 0;
+    int ncA = 0, nrec = 0;
     // analytic classes: the two record types, read once per pair
     int ta = 0, tb = 0;
     bool have = false, exhausted = false;
@@ -695,6 +699,7 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                             const int ncB = hb ? (int)B.p[2] : 0;
                             bounded = !cached && ncA + ncB > 0 &&
                                       4 * (ncA + ncB) <= cache_words<G>();
+                            nrec = ncA + ncB;
                             if (cached) {
                                 if (ha) cache_fill<G>(s, A, cache, 0, sub);
                                 if (hb) cache_fill<G>(s, B, cache, nA, sub);
@@ -745,7 +750,7 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                         w = support_pair_cached<G>(s, ia, ib, g.d, sub, cache, nA, two);
                     else if (bounded)
                         w = support_pair_bounds<G>(s, ia, ib, g.d, sub,
-                                                   reinterpret_cast<const float4 *>(cache), ncA);
+                                                   reinterpret_cast<const float4 *>(cache), ncA, nrec);
                     else
                         w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
                 } else if constexpr (!kHull) {
