@@ -183,7 +183,7 @@ def ptr(t):
 # work classes: size class 0..7 (total hull vertices 0, <=32, <=128, <=512,
 # <=1024, <=2048, <=4096, more), plus 8 when both colliders are hulls
 NUM_CLASSES = 16
-DEFAULT_LANES = (1, 4, 16, 8, 8, 16, 64, 64, 1, 8, 16, 16, 64, 16, 64, 64)
+DEFAULT_LANES = (1, 4, 16, 8, 16, 16, 32, 64, 1, 8, 16, 8, 64, 16, 64, 64)
 
 
 def expand_lanes(lanes):

@@ -819,9 +819,10 @@ __global__ __launch_bounds__(kQBlock) __attribute__((amdgpu_waves_per_eu(3))) vo
 
 // ---------------------------------------------------------------- host
 
-// the autotuner's table for the 1,048,576-pair mixed workload on one MI355X
-static const int kDefaultLanes[kNumClasses] = {1, 4, 16, 8, 8, 16, 64, 64,
-                                               1, 8, 16, 16, 64, 16, 64, 64};
+// the tuned table (distance3d_amd/tuning.json): lanes per class for the
+// 1,048,576-pair mixed workload on one MI355X, class 11 from the EPA pairs
+static const int kDefaultLanes[kNumClasses] = {1, 4, 16, 8, 16, 16, 32, 64,
+                                               1, 8, 16, 8, 64, 16, 64, 64};
 
 struct WsCache {
     std::mutex mu;

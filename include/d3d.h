@@ -115,7 +115,7 @@ typedef struct d3d_scene {
  *   5: <= 2048, 6: <= 4096, 7: > 4096;
  *   work class = size class, plus 8 when both colliders are hulls (D3D_NUM_CLASSES).
  * lanes_per_pair[c] lanes (a power of two <= 64; 0 = built-in default
- * 1,4,16,8,8,16,64,64 for one hull and 1,8,16,16,64,16,64,64 for two) work on
+ * 1,4,16,8,16,16,32,64 for one hull and 1,8,16,8,64,16,64,64 for two) work on
  * each pair of class c (class 0: 1, 2 or 4; larger values use 4).  For hull classes with 4 or more lanes, a pair whose hull
  * vertices fit the group's LDS slot (44, 85, 216, 456 or 1008 vertices with
  * 4, 8, 16, 32 or 64 lanes) has them copied there once and is swept from LDS;
