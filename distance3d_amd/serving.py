@@ -268,6 +268,33 @@ class CollisionServer:
         per_pair = 4 * 4 + 2 * (16 * 8 + 4 * 4) + 2 * 4 + (8 + 48 + 4 + 13 * 8 + 4)
         return slot_pairs * per_pair + kernels.gjk_epa_workspace_bytes(slot_pairs)
 
+    @classmethod
+    def slot_pairs_for(cls, budget_bytes, n_slots=4, granule=256, max_pairs=1 << 24):
+        """The largest slot size (pairs, a multiple of ``granule``) whose
+        ``n_slots`` slots fit in ``budget_bytes``; 0 when not even one granule
+        fits."""
+        lo, hi = 0, max_pairs // granule
+        while lo < hi:
+            mid = (lo + hi + 1) // 2
+            if n_slots * cls.slot_bytes(mid * granule) <= budget_bytes:
+                lo = mid
+            else:
+                hi = mid - 1
+        return lo * granule
+
+    @classmethod
+    def with_memory_fraction(cls, scene, poses, fraction=0.5, n_slots=4, **kw):
+        """A server whose slots take ``fraction`` of the device memory free now:
+        the memory budget that decides which requests a slot admits, sized
+        from the device instead of a fixed pair count."""
+        import torch
+        free, _ = torch.cuda.mem_get_info(scene.device)
+        pairs = cls.slot_pairs_for(int(free * fraction), n_slots)
+        if pairs == 0:
+            raise MemoryError("%.0f%% of the free device memory (%d bytes) holds no slot"
+                              % (100 * fraction, free))
+        return cls(scene, poses, n_slots=n_slots, slot_pairs=pairs, **kw)
+
     def serve(self, requests, realtime=True, clock=time.perf_counter):
         """Serve requests sorted by ``arrival`` (seconds from the start).
         Returns (results by request id, stats)."""

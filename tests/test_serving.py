@@ -137,3 +137,41 @@ def test_slot_input_layout_is_contiguous_per_bucket():
     assert slot.bucket == 1024
     h = slot.h_in.numpy()[:4 * 1024].reshape(4, 1024)
     assert np.array_equal(h[0, :300], np.arange(300)) and np.all(h[1, 300:] == 8)
+
+
+def test_slot_size_from_a_memory_budget():
+    """Slots sized from a byte budget: the largest multiple of 256 pairs whose
+    slots fit, monotone in the budget, and 0 when nothing fits."""
+    S = serving.CollisionServer
+    one = S.slot_bytes(256)
+    assert S.slot_pairs_for(4 * one - 1, n_slots=4) == 0
+    assert S.slot_pairs_for(4 * one, n_slots=4) == 256
+    prev = 0
+    for budget in (1 << 20, 1 << 24, 1 << 28, 1 << 32, 1 << 36):
+        p = S.slot_pairs_for(budget, n_slots=4)
+        assert p % 256 == 0 and p >= prev
+        assert 4 * S.slot_bytes(p) <= budget
+        if p < (1 << 24):
+            assert 4 * S.slot_bytes(p + 256) > budget
+        prev = p
+    assert S.slot_pairs_for(1 << 40, n_slots=4) == 1 << 24
+
+
+@pytest.mark.gpu
+def test_server_sized_from_free_memory(gpu):
+    """A server whose slots take a fraction of the free device memory admits
+    and serves like one sized by hand: the same results pair for pair."""
+    import torch
+    scene, poses, reqs = workloads.serving_workload(seed=12, n_colliders=64, n_requests=40,
+                                                    max_pairs=512)
+    dev = scene.to_device()
+    free, _ = torch.cuda.mem_get_info(dev.device)
+    want = serving.CollisionServer.slot_bytes(8192) * 2
+    srv = serving.CollisionServer.with_memory_fraction(dev, poses, fraction=want / free,
+                                                       n_slots=2)
+    assert 8192 - 256 <= srv.slots[0].capacity <= 8192
+    ref = serving.CollisionServer(dev, poses, n_slots=2, slot_pairs=4096)
+    got, _ = srv.serve(reqs, realtime=False)
+    exp, _ = ref.serve(reqs, realtime=False)
+    for r in reqs:
+        assert got[r.rid].same_as(exp[r.rid])
