@@ -287,6 +287,8 @@ class CollisionServer:
         """A server whose slots take ``fraction`` of the device memory free now:
         the memory budget that decides which requests a slot admits, sized
         from the device instead of a fixed pair count."""
+        if not 0.0 < fraction <= 1.0:
+            raise ValueError("fraction must be in (0, 1], got %r" % (fraction,))
         import torch
         free, _ = torch.cuda.mem_get_info(scene.device)
         pairs = cls.slot_pairs_for(int(free * fraction), n_slots)

@@ -155,6 +155,9 @@ def test_slot_size_from_a_memory_budget():
             assert 4 * S.slot_bytes(p + 256) > budget
         prev = p
     assert S.slot_pairs_for(1 << 40, n_slots=4) == 1 << 24
+    for bad in (0.0, -0.5, 1.5, float("nan")):
+        with pytest.raises(ValueError):
+            S.with_memory_fraction(None, None, fraction=bad)
 
 
 @pytest.mark.gpu
