@@ -9,7 +9,8 @@ where any difference in operation order would show.
 Bar: the GJK outputs (distance, closest points, status, iteration count)
 and the intersection flags equal the FP64 oracle's bit for bit for every
 lanes-per-pair value; EPA on the intersecting pairs gives the oracle's
-status, iteration count, depth and normal bit for bit.
+status, iteration count, depth and normal bit for bit; MPR its status and
+penetration outputs bit for bit.
 """
 import numpy as np
 import pytest
@@ -129,3 +130,39 @@ def test_degenerate_geometry_epa_bitwise(gpu):
     assert np.array_equal(_np(r.depth), o["depth"])
     conv = o["status"] == 1
     assert np.array_equal(_np(r.normal)[conv], o["normal"][conv])
+
+
+@pytest.mark.gpu
+def test_degenerate_geometry_mpr_bitwise(gpu):
+    """MPR from the colliders' centres (coincident for a collider against
+    itself): status and, with penetration info, depth, direction and contact
+    bit for bit."""
+    from distance3d_amd.scene import collider_centers
+    host, pa, pb = degenerate_scene()
+    centers = collider_centers(host)
+    dev = host.to_device()
+    for penetration in (True, False):
+        st, out = kernels.mpr_batch(dev, pa, pb, centers, penetration=penetration)
+        est, eout = oracle.mpr(host, pa, pb, centers, penetration=penetration)
+        assert np.array_equal(st.cpu().numpy(), est)
+        if penetration:
+            assert np.array_equal(out.cpu().numpy(), eout, equal_nan=True)
+            assert np.isfinite(eout[est == 1]).all() and np.sum(est == 1) > 400
+
+
+@pytest.mark.gpu
+def test_degenerate_geometry_aabbs_and_overlaps_exact(gpu):
+    """AABBs of the degenerate colliders (zero extent along some axes) bit for
+    bit, and the overlap pairs among them, touching boxes included, equal to
+    the brute-force closed-interval set."""
+    host, _, _ = degenerate_scene()
+    ids = np.arange(host.n_shapes, dtype=np.int32)
+    aabbs = kernels.aabb_batch(host.to_device(), ids)
+    host_aabbs = oracle.aabbs(host, ids)
+    assert np.array_equal(aabbs.cpu().numpy(), host_aabbs)
+    bvh = kernels.bvh_build(aabbs)
+    g = kernels.bvh_query(bvh, aabbs).cpu().numpy().astype(np.int64)
+    o, cnt = oracle.all_aabbs_overlap(host_aabbs, host_aabbs)
+    assert len(o) == cnt
+    o = o[np.lexsort((o[:, 1], o[:, 0]))]
+    assert np.array_equal(g, o)
