@@ -261,24 +261,62 @@ def test_update_collider_poses_refits(gpu):
 
 
 def test_self_collision_detect(gpu):  # self_collision.py:5-64
+    """The batched pass against the reference's per-frame loop run here with
+    the drop-in's per-frame tree query (aabb_overlapping_colliders) and the
+    oracle's GJK intersection: whitelists holding the frame and its chain
+    neighbours, some one-sided, one frame left its own candidate."""
     rs = np.random.RandomState(6)
     tm, bvh = _robotish(rs, 24)
-    for f in bvh.colliders_:
-        bvh.self_collision_whitelists_[f] = ()
-    contacts = self_collision.detect(bvh)
-    # brute force: every pair through the checker's GJK intersection
-    b = SceneBuilder()
     frames = list(bvh.colliders_)
-    ids = [bvh.colliders_[f].pack(b) for f in frames]
+    b = SceneBuilder()
+    ids = {f: bvh.colliders_[f].pack(b) for f in frames}
     host = b.build()
-    pa, pb = np.triu_indices(len(frames), 1)
-    hit, _ = oracle.gjk_intersection(host, np.array(ids)[pa], np.array(ids)[pb])
-    expect = {f: False for f in frames}
-    for i, j, h in zip(pa, pb, hit):
-        if h:
-            expect[frames[i]] = expect[frames[j]] = True
-    assert contacts == expect
-    assert self_collision.detect_any(bvh) == any(expect.values())
+
+    def intersects(f, f2):
+        hit, _ = oracle.gjk_intersection(host, [ids[f]], [ids[f2]])
+        return bool(hit[0])
+
+    def reference_detect():
+        contacts = {}
+        for f, c in bvh.colliders_.items():
+            if f in contacts:
+                continue
+            cand = bvh.aabb_overlapping_colliders(c, whitelist=bvh.self_collision_whitelists_[f])
+            contacts[f] = False
+            for f2 in cand:
+                if intersects(f, f2):
+                    contacts[f] = contacts[f2] = True
+                    break
+        return contacts
+
+    def reference_detect_any():
+        for f, c in bvh.colliders_.items():
+            cand = bvh.aabb_overlapping_colliders(c, whitelist=bvh.self_collision_whitelists_[f])
+            if any(intersects(f, f2) for f2 in cand):
+                return True
+        return False
+
+    for k, f in enumerate(frames):
+        wl = [f]
+        if k + 1 < len(frames):
+            wl.append(frames[k + 1])
+        if k > 0 and k % 3 != 0:  # every third frame does not whitelist its predecessor
+            wl.append(frames[k - 1])
+        bvh.self_collision_whitelists_[f] = tuple(wl)
+    for variant in range(2):
+        want = reference_detect()
+        got = self_collision.detect(bvh)
+        assert got == want and list(got) == list(want), variant
+        assert 0 < sum(want.values()) < len(frames)
+        assert self_collision.detect_any(bvh) == reference_detect_any()
+        if variant == 0:
+            # a frame that is its own candidate touches itself
+            quiet = next(f for f in frames if not got[f])
+            bvh.self_collision_whitelists_[quiet] = ()
+    assert got[quiet]
+    del bvh.self_collision_whitelists_[frames[0]]
+    with pytest.raises(KeyError):
+        self_collision.detect(bvh)
 
 
 def test_cube_sphere_unique_pairs_edge_case(gpu):  # test_cube_unique_pairs_edge_case.py:8-31
