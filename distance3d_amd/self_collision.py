@@ -12,7 +12,11 @@ reference's loop is then replayed on the host over those results, so the
 outputs are its outputs: a frame is a candidate of itself unless its
 whitelist holds it, a frame already marked is skipped, the first hit ends a
 frame's candidates, the contacts dict is filled in the same order, and a
-frame without a whitelist raises KeyError where the reference would.
+frame without a whitelist raises KeyError where the reference would.  A
+frame's candidates come in ascending leaf order, the order
+AabbTree.overlaps_aabb returns here (the reference's depth-first order
+depends on its tree layout); with symmetric whitelists, as urdf_utils builds
+them, which of several hits ends a frame changes no contact flag.
 """
 import numpy as np
 
