@@ -15,8 +15,9 @@ frame's candidates, the contacts dict is filled in the same order, and a
 frame without a whitelist raises KeyError where the reference would.  A
 frame's candidates come in ascending leaf order, the order
 AabbTree.overlaps_aabb returns here (the reference's depth-first order
-depends on its tree layout); with symmetric whitelists, as urdf_utils builds
-them, which of several hits ends a frame changes no contact flag.
+depends on its tree layout); with symmetric whitelists (each frame
+whitelists another exactly when that one whitelists it back), which of
+several hits ends a frame changes no contact flag.
 """
 import numpy as np
 
