@@ -201,6 +201,59 @@ def test_hull_hull_vs_minkowski_brute_force():
     assert n_sep > 30
 
 
+def _flat_pair(rs):
+    """Two small hulls whose Minkowski difference is flat (rank <= 2): both in
+    one plane, both on parallel lines, a point against a flat hull, or two
+    points."""
+    kind = rs.randint(4)
+    n1, n2 = rs.randint(1, 5), rs.randint(1, 5)
+    if kind == 0:  # both in planes z = c1, z = c2, half the time one plane
+        c1 = rs.randn()
+        c2 = c1 if rs.rand() < 0.5 else rs.randn() * 0.2
+        v1 = np.column_stack([rs.randn(n1, 2) * 0.6, np.full(n1, c1)])
+        v2 = np.column_stack([rs.randn(n2, 2) * 0.6 + rs.randn(2) * 0.5, np.full(n2, c2)])
+    elif kind == 1:  # on parallel lines
+        u = rs.randn(3)
+        v1 = np.outer(rs.randn(n1), u) + rs.randn(3) * 0.5
+        v2 = np.outer(rs.randn(n2), u) + rs.randn(3) * 0.5
+    elif kind == 2:  # a point against a flat hull
+        v1 = rs.randn(1, 3) * 0.7
+        v2 = np.column_stack([rs.randn(n2, 2) * 0.6, np.zeros(n2)])
+    else:  # two points
+        v1, v2 = rs.randn(1, 3), rs.randn(1, 3)
+    R = gb.random_rotation(rs)
+    return v1 @ R.T, v2 @ R.T
+
+
+def test_flat_minkowski_difference_vs_triangle_enumeration():
+    """Pairs whose Minkowski difference is flat (Qhull refuses it): the
+    distance is the smallest origin-to-triangle distance over all triples of
+    difference points (a flat convex set is the union of its triangles),
+    each by subset enumeration; zero when the origin lies in the set."""
+    import itertools
+    rs = np.random.RandomState(21)
+    n_sep = n_hit = 0
+    for _ in range(240):
+        v1, v2 = _flat_pair(rs)
+        md = (v1[:, None, :] - v2[None, :, :]).reshape(-1, 3)
+        centred = md - md.mean(axis=0)
+        assert np.linalg.matrix_rank(centred, tol=1e-9) <= 2
+        if len(md) == 1:
+            bd = float(np.linalg.norm(md[0]))
+        else:
+            bd = min(np.linalg.norm(gb.closest_point_of_points(md[list(t)]))
+                     for t in itertools.combinations(range(len(md)), min(3, len(md))))
+        d, cp1, cp2, st = _dist(_hull(v1), _hull(v2))
+        if bd > 1e-9:
+            n_sep += 1
+            assert d == approx(bd, rel=1e-9, abs=1e-12)
+            assert np.linalg.norm(cp1 - cp2) == approx(d, rel=1e-9, abs=1e-12)
+        else:
+            n_hit += 1
+            assert d <= 1e-9
+    assert n_sep > 150 and n_hit > 5
+
+
 def test_posed_hull_equals_world_hull():
     """A hull given in a local frame with a pose has the support of the
     world-frame vertex set (the reference's ConvexHullVertices semantics)."""
