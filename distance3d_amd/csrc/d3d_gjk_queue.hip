@@ -745,6 +745,10 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
             } else {
                 // the records were validated when the pair was taken
                 SupportPair w;
+                // a wavefront in its support sweep (memory rounds) issues ahead of
+                // those in their FP64 simplex step, so its loads go out sooner;
+                // priority only orders issue between the SIMD's wavefronts
+                __builtin_amdgcn_s_setprio(2);
                 if constexpr (kCache) {
                     if (cached)
                         w = support_pair_cached<G>(s, ia, ib, g.d, sub, cache, nA, two);
@@ -758,6 +762,7 @@ __device__ __forceinline__ void run_class(const d3d_scene &s, const int32_t *__r
                 } else {
                     w = support_pair<G, kHull>(s, ia, ib, g.d, sub);
                 }
+                __builtin_amdgcn_s_setprio(0);
                 running = gjk_distance_step(g, pq, w.a, w.b, prm.max_distance_squared, tol2);
             }
             if (!running) {
