@@ -274,3 +274,34 @@ def test_intersect_halfplanes_dropin(gpu):
         oracle.intersect_halfplanes(star)
     with pytest.raises(AssertionError):
         hc.intersect_halfplanes(star)
+
+
+def test_polygon_centre_summation_order_does_not_change_the_order():
+    """order_points (_tetrahedron_intersection.py:299-318) sorts a contact
+    polygon's points by angle around their mean.  The oracle and the kernel
+    sum the mean one element after another, as numba's np.mean does; numpy's
+    own np.mean sums pairwise from 8 elements on, and the two centres can
+    differ in the last bits.  For convex polygons of 8 to 23 points in general
+    position the angular order around either centre is the same, so the
+    summation order cannot change the ordered polygon there."""
+    rs = np.random.RandomState(17)
+    differ = 0
+    for _ in range(3000):
+        n = rs.randint(8, 24)
+        t = np.sort(rs.uniform(-np.pi, np.pi, n))
+        axes = rs.uniform(0.1, 2.0, 2)
+        pts = np.column_stack([axes[0] * np.cos(t), axes[1] * np.sin(t)])
+        a = rs.uniform(0, 2 * np.pi)
+        R = np.array([[np.cos(a), -np.sin(a)], [np.sin(a), np.cos(a)]])
+        pts = pts @ R.T + rs.uniform(-5, 5, 2)
+        pts = pts[rs.permutation(n)]
+        seq = np.zeros(2)
+        for p in pts:
+            seq = seq + p
+        seq = seq / n
+        pair = np.array([np.mean(pts[:, 0]), np.mean(pts[:, 1])])
+        differ += int(np.any(seq != pair))
+        a_seq = np.arctan2(pts[:, 1] - seq[1], pts[:, 0] - seq[0])
+        a_pair = np.arctan2(pts[:, 1] - pair[1], pts[:, 0] - pair[0])
+        assert np.array_equal(np.argsort(a_seq, kind="stable"), np.argsort(a_pair, kind="stable"))
+    assert differ > 300  # the centres do differ, often
